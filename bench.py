@@ -1,0 +1,149 @@
+"""Throughput benchmark: 18-DOF FP64 vehicle-steps/s on 1,000,000 vehicles (one MI355X).
+
+Workload (knot_fleet): HMMWV vehicle + TMeasy tires, dt = 1e-3 s, mass / yaw inertia / cornering
+stiffness jittered x U(0.9, 1.1) per vehicle, throttle piecewise constant U(0, 1) per 1 s,
+steering U(-0.3, 0.3) rad knots every 0.5 s (linear), brake U(0, 0.5) on a seeded 10 % of the
+1 s intervals.  One bench step integrates the whole fleet over a 10 s horizon (10,000 solver
+steps) in one launch, with every input already resident in HBM.
+
+Prints one JSON line; `roofline` uses the algorithmic FP64 FLOPs per vehicle-step counted by the
+oracle's counting scalar on the same workload (see DESIGN.md for the counting rule) and the
+device time of the integration launch from HIP events on the solver's stream.  `cpu_baseline`
+times the oracle (the CPU restatement, bit-identical to the reference solver) with OpenMP over
+vehicles on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "low-fidelity-dynamic-models_amd"))
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector / matrix peak
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+
+
+def flops_per_step(flat, nsteps=10000, nveh=4, seed=2):
+    """Algorithmic FLOPs per vehicle-step: + - * / sqrt count 1, sin/cos/tan/atan2/tanh count 20,
+    hypot counts 4 (oracle counting scalar over the knot_fleet workload)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lfdm_oracle as O
+    from lfdm_amd import workloads as W
+
+    fl = W.knot_fleet(nveh, flat, seed=seed)
+    basic = trans = 0
+    for i in range(nveh):
+        knots = fl["knots"][:, i:i + 1]
+        c = O.count_ops18(flat, ("knots", fl["ns"], fl["ds"], fl["np"], fl["dp"], knots), nsteps)
+        basic += c["add"] + c["mul"] + c["div"] + c["sqrt"]
+        trans += 20 * (c["sin"] + c["cos"] + c["tan"] + c["atan2"] + c["tanh"]) + 4 * c["hypot"]
+    steps = nsteps * nveh
+    return (basic + trans) / steps, basic / steps
+
+
+def cpu_baseline(flat, n_sample, nsteps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lfdm_oracle as O
+    from lfdm_amd import workloads as W
+
+    fl = W.knot_fleet(n_sample, flat, seed=2)
+    ctrl = ("knots", fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
+    t0 = time.perf_counter()
+    O.run18(flat, ctrl, n_sample, nsteps, ovr=fl["ovr"], threads=threads)
+    dt = time.perf_counter() - t0
+    # single-core rate on a small subset
+    n1 = 64
+    fl1 = W.knot_fleet(n1, flat, seed=3)
+    t1 = time.perf_counter()
+    O.run18(flat, ("knots", fl1["ns"], fl1["ds"], fl1["np"], fl1["dp"], fl1["knots"]), n1, nsteps, ovr=fl1["ovr"],
+            threads=1)
+    d1 = time.perf_counter() - t1
+    return n_sample * nsteps / dt, n1 * nsteps / d1, dt
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--vehicles", type=int, default=1_000_000)
+    ap.add_argument("--horizon-steps", type=int, default=10000)
+    ap.add_argument("--tpb", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; vehicles shard across GPUs with no "
+                                   "communication (see DESIGN.md), run with --gpus 1"}))
+        sys.exit(2)
+
+    import lfdm_amd
+    from lfdm_amd import SolverHalfImplicitGPU, TMeasy
+    from lfdm_amd import workloads as W
+
+    n, H = args.vehicles, args.horizon_steps
+    flat = lfdm_amd.params_from_json(18, W.HMMWV_VEH, W.HMMWV_TIRE)
+    fl = W.knot_fleet(n, flat, seed=2)
+    s = SolverHalfImplicitGPU(18, n)
+    s.construct_flat(flat, n, None, TMeasy)
+    s.Initialize(s.zero_record(), n)
+    s.set_overrides(fl["ovr"])
+    s.set_controls_knots(fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
+    s.SetThreadsPerBlock(args.tpb)
+    del fl
+    # warmup (also uploads the fleet)
+    for _ in range(args.warmup):
+        s.advance(H)
+    s.synchronize()
+    launch_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.advance(H)
+        launch_ms.append(s.last_launch_ms())  # waits for this launch's end event
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = n * H * args.steps / wall
+    kernel_ms = float(np.mean(launch_ms))
+    fl_total, fl_basic = flops_per_step(flat)
+    achieved_tf = fl_total * n * H / (kernel_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
+                "kernel": "d18_integrate<TMeasy, knots>", "kernel_ms": kernel_ms,
+                "flop_per_vehicle_step": fl_total, "basic_flop_per_vehicle_step": fl_basic}
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        rate, rate1, secs = cpu_baseline(flat, args.cpu_sample, H, threads)
+        cpu = {"value": rate, "unit": "vehicle-steps/s", "cores": threads, "kind": "port",
+               "sample": f"knot_fleet, {args.cpu_sample} vehicles x {H} steps ({secs:.1f} s), "
+                         f"oracle (bit-identical CPU restatement), OpenMP over vehicles",
+               "single_core": rate1, "cpu": cpu_model()}
+    line = {"metric": "18-DOF FP64 vehicle-steps/s at 1M vehicles", "value": value, "unit": "vehicle-steps/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "knot_fleet: 18-DOF HMMWV TMeasy, 1,000,000 vehicles, 10 s at dt=1e-3, "
+                                   "+-10% mass/Izz/cornering stiffness, knot controls",
+                       "vehicles": n, "horizon_steps": H, "dt": 1e-3, "threads_per_block": args.tpb},
+            "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line))
+    s.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,134 @@
+/* C ABI of the MI355X batched half-implicit vehicle solver (11/18/24-DOF, TMeasy / TMeasyNr).
+ *
+ * Drop-in boundary for the reference's GPU solver classes d18SolverHalfImplicitGPU
+ * (wheeled_vehicle_models/18dof-gpu/dof18_halfImplicit_gpu.cuh:15-343) and their 11/24-DOF twins.
+ * One opaque handle replaces one solver object; every call returns a status code (LFDM_OK or a
+ * negative LFDM_E_*) and lfdm_last_error() describes the last failure on the calling thread.
+ * Plain pointers and sizes only: host buffers unless a name says "device".
+ *
+ * Records: a vehicle's full state is exchanged as a flat array of doubles in a fixed order
+ * (lfdm_record_len gives its length):
+ *   18-DOF, per tire LF RF LR RR, TMeasy  : xe ye xedot yedot omega dOmega xt rStat fx fy fz vsx vsy My engTor
+ *                                 TMeasyNr: omega dOmega xt rStat fx fy fz vsx vsy My engTor
+ *           then the vehicle              : x y u v psi wz phi wx udot vdot wxdot wzdot crankOmega dOmega_crank gear
+ */
+#ifndef LFDM_AMD_H
+#define LFDM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LFDM_OK 0
+#define LFDM_E_ARG (-1)         /* invalid argument or handle */
+#define LFDM_E_IO (-2)          /* file cannot be opened or holds no data */
+#define LFDM_E_PARSE (-3)       /* malformed JSON or missing / mistyped key */
+#define LFDM_E_STATE (-4)       /* call out of order (e.g. Solve before SetEndTime) */
+#define LFDM_E_CAPACITY (-5)    /* more vehicles than the handle was created for, or a map too large */
+#define LFDM_E_HIP (-6)         /* HIP runtime error (no GPU, launch failure, out of memory) */
+#define LFDM_E_UNSUPPORTED (-7) /* model / option not available */
+
+#define LFDM_TIRE_TMEASY 0   /* TireType::TMeasy   (dof18_gpu.cuh, enum TireType) */
+#define LFDM_TIRE_TMEASYNR 1 /* TireType::TMeasyNr */
+
+typedef struct lfdm_solver lfdm_solver;
+
+/* Text of the last error on this thread ("" if none). */
+const char* lfdm_last_error(void);
+
+/* Length of the full state record of one vehicle. */
+int lfdm_record_len(int dof, int tire_type);
+
+/* ---- setup helpers (host only, no GPU needed) ------------------------------------------ */
+
+/* Parses vehicle + tire (+ suspension for 24-DOF) JSON into the flat parameter vector the
+ * handle and the fixtures exchange; replaces setVehParamsJSON / setTireParamsJSON / tireInit
+ * (18dof-gpu/dof18_gpu.cu:1075-1460).  *len receives the length; fails with LFDM_E_CAPACITY if
+ * cap is too small. */
+int lfdm_params_from_json(int dof, const char* veh_json, const char* tire_json, const char* sus_json, int tire_type,
+                          double* out, int cap, int* len);
+
+/* Loads a driver-input table "time steering throttle braking" (LoadDriverData, utils_gpu.cu:7-31)
+ * into rows[nrows][4]. */
+int lfdm_driver_file_load(const char* path, double* rows, int cap_rows, int* nrows);
+
+/* ---- solver handle: the reference class API ------------------------------------------- */
+
+/* d18SolverHalfImplicitGPU(unsigned total_num_vehicles), dof18_halfImplicit_gpu.cuh:19 */
+int lfdm_create(int dof, unsigned total_vehicles, lfdm_solver** out);
+/* ~d18SolverHalfImplicitGPU(), .cuh:20 */
+int lfdm_destroy(lfdm_solver* h);
+
+/* Construct(veh, tire, n[, driver][, TireType]), .cuh:34-87; sus_json only for 24-DOF; driver_file
+ * may be NULL (controller-driven fleets).  Appends a group of n vehicles. */
+int lfdm_construct(lfdm_solver* h, const char* veh_json, const char* tire_json, const char* sus_json,
+                   unsigned n, const char* driver_file, int tire_type);
+/* Same from a flat parameter vector and an in-memory driver table rows[nrows][4] (may be NULL). */
+int lfdm_construct_flat(lfdm_solver* h, const double* flat, int len, int tire_type, unsigned n,
+                        const double* rows, int nrows);
+
+/* Initialize(states..., n), .cuh:172-195: appends n vehicles with the given full record. */
+int lfdm_initialize(lfdm_solver* h, const double* record, unsigned n);
+
+/* Setters / getters, .cuh:93-129 */
+int lfdm_set_end_time(lfdm_solver* h, double tend);
+int lfdm_set_time_step(lfdm_solver* h, double step);
+int lfdm_set_kernel_sim_time(lfdm_solver* h, double time);
+int lfdm_set_host_dump_time(lfdm_solver* h, double time);
+int lfdm_set_threads_per_block(lfdm_solver* h, unsigned threads);
+int lfdm_get_step(lfdm_solver* h, double* step);
+int lfdm_get_end_time(lfdm_solver* h, double* tend);
+int lfdm_get_tire_type(lfdm_solver* h, int* tire_type);
+
+/* SetOutput(file, freq, store_all, no_outs), .cuh:138-141: CSV files <file>_<vehicle>.csv */
+int lfdm_set_output(lfdm_solver* h, const char* output_file, double output_freq, int store_all, unsigned no_outs);
+
+/* Solve(), .cuh:148: integrates every vehicle to the end time; returns after the GPU finished. */
+int lfdm_solve(lfdm_solver* h);
+
+/* SolveStep(t, steering, throttle, braking), .cuh:158: one kernel-sim-time chunk with uniform
+ * controls; asynchronous; *t_out = t + kernel_sim_time. */
+int lfdm_solve_step(lfdm_solver* h, double t, double steering, double throttle, double braking, double* t_out);
+
+/* GetSimState(idx), .cuh:204: full record of one vehicle (synchronizes). */
+int lfdm_get_sim_state(lfdm_solver* h, unsigned vehicle, double* record);
+
+/* ---- batched extensions (no counterpart in the reference) ------------------------------ */
+
+/* All records at once, out[total][record] (synchronizes). */
+int lfdm_get_states(lfdm_solver* h, double* records);
+/* Replaces the records of all vehicles, in[total][record]. */
+int lfdm_set_states(lfdm_solver* h, const double* records);
+
+/* Per-vehicle parameter values ovr[7][total]: m, jz, jx, dfy0Pn, dfy0P2n, torque_scale (multiplies
+ * the engine torque-map ordinates), maxBrakeTorque.  Defaults come from each vehicle's group. */
+int lfdm_set_overrides(lfdm_solver* h, const double* ovr);
+
+/* Control sources for Solve (the default is each group's driver table):
+ *  knots: steering linear on a grid of ds seconds (ns knots), throttle and braking piecewise
+ *         constant on a grid of dp seconds (np values each); knots[ns + 2 np][total].
+ *  steps: one (steering, throttle, braking) per step, ctrl[nsteps][3][total]; host or device. */
+int lfdm_set_controls_knots(lfdm_solver* h, int ns, double ds, int np, double dp, const double* knots);
+int lfdm_set_controls_steps(lfdm_solver* h, int nsteps, const double* ctrl);
+int lfdm_set_controls_steps_device(lfdm_solver* h, int nsteps, const double* device_ctrl);
+int lfdm_set_controls_table(lfdm_solver* h);
+
+/* Advances every vehicle nsteps steps from its current time in one launch (no output), on the
+ * handle's stream, without synchronizing.  The time base and the position in per-step controls
+ * continue across calls. */
+int lfdm_advance(lfdm_solver* h, int nsteps);
+/* Resets the handle's time base to t and the per-step control position to row 0 (Solve always
+ * starts at t = 0 from row 0). */
+int lfdm_set_time(lfdm_solver* h, double t);
+
+/* Stream used for every launch and copy (hipStream_t; NULL = the handle's own stream). */
+int lfdm_set_stream(lfdm_solver* h, void* stream);
+int lfdm_synchronize(lfdm_solver* h);
+/* Device time of the last integration launch, from HIP events on the handle's stream. */
+int lfdm_last_launch_ms(lfdm_solver* h, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LFDM_AMD_H */

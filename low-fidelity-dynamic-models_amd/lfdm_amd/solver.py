@@ -1,0 +1,213 @@
+"""Python mirror of the reference GPU solver classes over the C ABI.
+
+``SolverHalfImplicitGPU(dof, total)`` follows d18SolverHalfImplicitGPU
+(wheeled_vehicle_models/18dof-gpu/dof18_halfImplicit_gpu.cuh:15-343): Construct, Initialize,
+the Set*/Get* methods, SetOutput, Solve, SolveStep and GetSimState keep the reference's names,
+argument order and meaning.  States are exchanged as full records (numpy float64 arrays in the
+order documented in include/lfdm_amd.h); batched methods move whole fleets at once.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import LFDM_TIRE_TMEASY, LFDM_TIRE_TMEASYNR, check, enc, lib
+
+TMeasy = LFDM_TIRE_TMEASY
+TMeasyNr = LFDM_TIRE_TMEASYNR
+
+# Field names of the canonical 18-DOF record (include/lfdm_amd.h).
+TIRE_FIELDS = ("xe", "ye", "xedot", "yedot", "omega", "dOmega", "xt", "rStat", "fx", "fy", "fz", "vsx", "vsy",
+               "My", "engTor")
+TIRE_FIELDS_NR = TIRE_FIELDS[4:]
+VEH_FIELDS = ("x", "y", "u", "v", "psi", "wz", "phi", "wx", "udot", "vdot", "wxdot", "wzdot", "crankOmega",
+              "dOmega_crank", "current_gr")
+TIRES = ("lf", "rf", "lr", "rr")
+OVERRIDE_FIELDS = ("m", "jz", "jx", "dfy0Pn", "dfy0P2n", "torque_scale", "maxBrakeTorque")
+
+
+def record_fields(dof=18, tire_type=TMeasy):
+    """Names of the record entries, e.g. 'lf.omega' or 'x'."""
+    if dof != 18:
+        raise ValueError(f"unsupported model: {dof}-DOF")
+    tf = TIRE_FIELDS_NR if tire_type == TMeasyNr else TIRE_FIELDS
+    return [f"{t}.{f}" for t in TIRES for f in tf] + list(VEH_FIELDS)
+
+
+def record_index(name, dof=18, tire_type=TMeasy):
+    return record_fields(dof, tire_type).index(name)
+
+
+def record_len(dof=18, tire_type=TMeasy):
+    return check(lib.lfdm_record_len(dof, tire_type))
+
+
+def params_from_json(dof, veh_json, tire_json, sus_json=None, tire_type=TMeasy):
+    """Flat parameter vector (setVehParamsJSON + setTireParamsJSON + tireInit semantics)."""
+    buf = np.zeros(8192)
+    n = ctypes.c_int(0)
+    check(lib.lfdm_params_from_json(dof, enc(veh_json), enc(tire_json), enc(sus_json), tire_type,
+                                    buf.ctypes.data, buf.size, ctypes.byref(n)))
+    return buf[: n.value].copy()
+
+
+def load_driver_file(path):
+    """Driver table rows (time, steering, throttle, braking) as an (n, 4) array."""
+    cap = 1 << 16
+    rows = np.zeros((cap, 4))
+    n = ctypes.c_int(0)
+    check(lib.lfdm_driver_file_load(enc(path), rows.ctypes.data, cap, ctypes.byref(n)))
+    return rows[: n.value].copy()
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class SolverHalfImplicitGPU:
+    """One solver handle; see the module docstring."""
+
+    def __init__(self, dof, total_num_vehicles):
+        h = ctypes.c_void_p()
+        check(lib.lfdm_create(dof, total_num_vehicles, ctypes.byref(h)))
+        self._h = h
+        self.dof = dof
+        self.total = total_num_vehicles
+        self.tire_type = None
+
+    def close(self):
+        if self._h:
+            lib.lfdm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- construction ------------------------------------------------------------------
+    def Construct(self, veh_params_file, tire_params_file, num_vehicles, driver_file=None, tire_type=TMeasy,
+                  sus_params_file=None):
+        check(lib.lfdm_construct(self._h, enc(veh_params_file), enc(tire_params_file), enc(sus_params_file),
+                                 num_vehicles, enc(driver_file), tire_type))
+        self.tire_type = tire_type
+
+    def construct_flat(self, flat, num_vehicles, driver_rows=None, tire_type=TMeasy):
+        flat = _f64(flat)
+        rows = None if driver_rows is None else _f64(driver_rows)
+        check(lib.lfdm_construct_flat(self._h, flat.ctypes.data, flat.size, tire_type, num_vehicles,
+                                      None if rows is None else rows.ctypes.data,
+                                      0 if rows is None else rows.shape[0]))
+        self.tire_type = tire_type
+
+    def Initialize(self, record, num_vehicles):
+        rec = _f64(record)
+        check(lib.lfdm_initialize(self._h, rec.ctypes.data, num_vehicles))
+
+    def zero_record(self):
+        return np.zeros(record_len(self.dof, self.tire_type or TMeasy))
+
+    # -- settings ----------------------------------------------------------------------
+    def SetEndTime(self, tend):
+        check(lib.lfdm_set_end_time(self._h, tend))
+
+    def SetTimeStep(self, step):
+        check(lib.lfdm_set_time_step(self._h, step))
+
+    def SetKernelSimTime(self, t):
+        check(lib.lfdm_set_kernel_sim_time(self._h, t))
+
+    def SetHostDumpTime(self, t):
+        check(lib.lfdm_set_host_dump_time(self._h, t))
+
+    def SetThreadsPerBlock(self, n):
+        check(lib.lfdm_set_threads_per_block(self._h, n))
+
+    def GetStep(self):
+        v = ctypes.c_double()
+        check(lib.lfdm_get_step(self._h, ctypes.byref(v)))
+        return v.value
+
+    def GetEndTime(self):
+        v = ctypes.c_double()
+        check(lib.lfdm_get_end_time(self._h, ctypes.byref(v)))
+        return v.value
+
+    def GetTireType(self):
+        v = ctypes.c_int()
+        check(lib.lfdm_get_tire_type(self._h, ctypes.byref(v)))
+        return v.value
+
+    def SetOutput(self, output_file, output_freq, store_all=False, no_outs=50):
+        check(lib.lfdm_set_output(self._h, enc(output_file), output_freq, int(store_all), no_outs))
+
+    # -- integration -------------------------------------------------------------------
+    def Solve(self):
+        check(lib.lfdm_solve(self._h))
+
+    def SolveStep(self, t, steering, throttle, braking):
+        out = ctypes.c_double()
+        check(lib.lfdm_solve_step(self._h, t, steering, throttle, braking, ctypes.byref(out)))
+        return out.value
+
+    def GetSimState(self, vehicle_index):
+        rec = self.zero_record()
+        check(lib.lfdm_get_sim_state(self._h, vehicle_index, rec.ctypes.data))
+        return rec
+
+    # -- batched extensions -------------------------------------------------------------
+    def get_states(self):
+        out = np.zeros((self.total, record_len(self.dof, self.tire_type or TMeasy)))
+        check(lib.lfdm_get_states(self._h, out.ctypes.data))
+        return out
+
+    def set_states(self, records):
+        rec = _f64(records)
+        check(lib.lfdm_set_states(self._h, rec.ctypes.data))
+
+    def set_overrides(self, ovr):
+        """ovr: (7, total) array of m, jz, jx, dfy0Pn, dfy0P2n, torque_scale, maxBrakeTorque."""
+        ovr = _f64(ovr)
+        assert ovr.shape == (len(OVERRIDE_FIELDS), self.total)
+        check(lib.lfdm_set_overrides(self._h, ovr.ctypes.data))
+
+    def set_controls_knots(self, ns, ds, np_, dp, knots):
+        knots = _f64(knots)
+        assert knots.shape == (ns + 2 * np_, self.total)
+        check(lib.lfdm_set_controls_knots(self._h, ns, ds, np_, dp, knots.ctypes.data))
+
+    def set_controls_steps(self, ctrl):
+        """ctrl: (nsteps, 3, total) steering, throttle, braking per step (host array)."""
+        ctrl = _f64(ctrl)
+        assert ctrl.ndim == 3 and ctrl.shape[1:] == (3, self.total)
+        self._steps_keep = ctrl
+        check(lib.lfdm_set_controls_steps(self._h, ctrl.shape[0], ctrl.ctypes.data))
+
+    def set_controls_steps_device(self, nsteps, device_ptr):
+        check(lib.lfdm_set_controls_steps_device(self._h, nsteps, ctypes.c_void_p(device_ptr)))
+
+    def set_controls_table(self):
+        check(lib.lfdm_set_controls_table(self._h))
+
+    def advance(self, nsteps):
+        check(lib.lfdm_advance(self._h, nsteps))
+
+    def set_time(self, t):
+        check(lib.lfdm_set_time(self._h, t))
+
+    def set_stream(self, stream_ptr):
+        check(lib.lfdm_set_stream(self._h, ctypes.c_void_p(stream_ptr)))
+
+    def synchronize(self):
+        check(lib.lfdm_synchronize(self._h))
+
+    def last_launch_ms(self):
+        v = ctypes.c_double()
+        check(lib.lfdm_last_launch_ms(self._h, ctypes.byref(v)))
+        return v.value

@@ -1,0 +1,170 @@
+// ORACLE — test infrastructure only. Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load liblfdm_oracle.so; the product library never links or calls it.
+//
+// Flat C entry points over the CPU restatement (orc18.h and siblings).  Every batch entry point
+// runs vehicles independently (OpenMP over vehicles, one vehicle's whole horizon per task).
+#include <omp.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "orc_count.h"
+#include "orc18.h"
+
+namespace orc {
+thread_local OpCount g_ops;
+}
+
+using namespace orc;
+
+namespace {
+
+// Per-vehicle override fields, in order: m, jz, jx, dfy0Pn, dfy0P2n, torque_scale, maxBrakeTorque.
+constexpr int kOverrides = 7;
+
+template <class T>
+void apply_overrides_18(d18::Vehicle<T>& s, const double* ovr, int n, int i) {
+    if (!ovr) return;
+    s.vp.m = T(ovr[0 * n + i]);
+    s.vp.jz = T(ovr[1 * n + i]);
+    s.vp.jx = T(ovr[2 * n + i]);
+    s.tp.dfy0Pn = T(ovr[3 * n + i]);
+    s.tp.dfy0P2n = T(ovr[4 * n + i]);
+    const double sc = ovr[5 * n + i];
+    for (auto& e : s.vp.torque) e.y = e.y * sc;
+    s.vp.maxBrakeTorque = T(ovr[6 * n + i]);
+}
+
+// Control sources. mode 0: per-step array ctrl[(k*3 + c)*n + i] (c = steering, throttle, braking);
+// mode 1: one driver table (rows of t, steering, throttle, braking; imeta[0] = rows);
+// mode 2: per-vehicle knot program (imeta = {ns, np}, dmeta = {ds, dp}; knots per vehicle
+//         laid out [ns + 2 np][n]).
+struct Controls {
+    int mode;
+    const double* ctrl;
+    const int* imeta;
+    const double* dmeta;
+    int n;
+    void at(int i, int k, double t, double& s, double& th, double& b) const {
+        if (mode == 0) {
+            s = ctrl[((size_t)k * 3 + 0) * n + i];
+            th = ctrl[((size_t)k * 3 + 1) * n + i];
+            b = ctrl[((size_t)k * 3 + 2) * n + i];
+        } else if (mode == 1) {
+            driver_input(t, reinterpret_cast<const DriverRow*>(ctrl), imeta[0], s, th, b);
+        } else {
+            const int ns = imeta[0], np = imeta[1];
+            const int w = ns + 2 * np;
+            double kn[256];
+            for (int j = 0; j < w && j < 256; ++j) kn[j] = ctrl[(size_t)j * n + i];
+            knot_input(t, kn, ns, dmeta[0], np, dmeta[1], s, th, b);
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int orc18_record_len(int nr) { return d18::record_len(nr != 0); }
+
+// Runs n vehicles of one 18-DOF parameter set for nsteps steps of dt from t0 (t += dt per step,
+// as dof18_halfImplicit.cpp:229).  init: [n][record] or NULL (default-constructed states).
+// out (optional): every `every` steps, the records of all vehicles, [sample][n][record].
+// fin (optional): final records [n][record].  Returns 0.
+int orc18_run(int nr, const double* flat, const double* ovr, const double* init, int n, double t0, double dt,
+              int nsteps, int mode, const double* ctrl, const int* imeta, const double* dmeta, int every, double* out,
+              double* fin, int threads) {
+    const int rl = d18::record_len(nr != 0);
+    Controls cs{mode, ctrl, imeta, dmeta, n};
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < n; ++i) {
+        d18::Vehicle<double> s;
+        s.nr = nr != 0;
+        d18::read_params(flat, s.nr, s.vp, s.tp);
+        apply_overrides_18(s, ovr, n, i);
+        if (init) {
+            d18::from_record(s, init + (size_t)i * rl);
+        } else {
+            std::vector<double> z(rl, 0.0);
+            d18::from_record(s, z.data());
+        }
+        double t = t0;
+        int rec = 0;
+        for (int k = 0; k < nsteps; ++k) {
+            double st, th, br;
+            cs.at(i, k, t, st, th, br);
+            d18::step(s, st, th, br, dt);
+            t = t + dt;
+            if (out && every > 0 && (k + 1) % every == 0) {
+                d18::to_record(s, out + ((size_t)rec * n + i) * rl);
+                ++rec;
+            }
+        }
+        if (fin) d18::to_record(s, fin + (size_t)i * rl);
+    }
+    return 0;
+}
+
+// Counts operations over nsteps of one vehicle (same arguments as orc18_run with n = 1).
+// counts: add, mul, div, sqrt, sin, cos, tan, atan2, hypot, tanh (totals over the run).
+int orc18_count_ops(int nr, const double* flat, const double* init, double t0, double dt, int nsteps, int mode,
+                    const double* ctrl, const int* imeta, const double* dmeta, long long* counts) {
+    const int rl = d18::record_len(nr != 0);
+    Controls cs{mode, ctrl, imeta, dmeta, 1};
+    d18::Vehicle<CD> s;
+    s.nr = nr != 0;
+    d18::read_params(flat, s.nr, s.vp, s.tp);
+    std::vector<double> z(rl, 0.0);
+    d18::from_record(s, init ? init : z.data());
+    g_ops = OpCount{};
+    double t = t0;
+    for (int k = 0; k < nsteps; ++k) {
+        double st, th, br;
+        cs.at(0, k, t, st, th, br);
+        d18::step(s, CD(st), CD(th), CD(br), dt);
+        t = t + dt;
+    }
+    const long long c[] = {g_ops.add, g_ops.mul,  g_ops.div,   g_ops.sqrt,  g_ops.sin,
+                           g_ops.cos, g_ops.tan,  g_ops.atan2, g_ops.hypot, g_ops.tanh};
+    std::memcpy(counts, c, sizeof(c));
+    return 0;
+}
+
+// ---- building blocks, exposed for closed-form and invariant checks ----
+
+void orc_tmxy(double s, double df0, double sm, double fm, double ss, double fs, double* f, double* fos) {
+    d18::tmxy_combined(*f, *fos, s, df0, sm, fm, ss, fs);
+}
+double orc_interp_l(double fz, double w1, double w2, double pn) { return interp_l(fz, w1, w2, pn); }
+double orc_interp_q(double fz, double w1, double w2, double pn) { return interp_q(fz, w1, w2, pn); }
+double orc_sine_step(double x, double x1, double y1, double x2, double y2) { return sine_step(x, x1, y1, x2, y2); }
+double orc_map_y(const double* xy, int n, double x) {
+    std::vector<MapE<double>> m(n);
+    for (int i = 0; i < n; ++i) m[i] = MapE<double>{xy[2 * i], xy[2 * i + 1]};
+    return map_y(m, x);
+}
+void orc_diff_split(double torque, double sl, double sr, double* tl, double* tr) {
+    d18::differential_split(torque, 2.0, sl, sr, *tl, *tr);
+}
+// Vertical loads of one vehicle state (record layout) for an 18-DOF parameter vector.
+void orc18_loads(int nr, const double* flat, const double* rec, double* loads) {
+    d18::Vehicle<double> s;
+    s.nr = nr != 0;
+    d18::read_params(flat, s.nr, s.vp, s.tp);
+    d18::from_record(s, rec);
+    d18::tire_loads(s, loads);
+}
+// Chassis accelerations (udot, vdot, wxdot, wzdot) for given tire forces in the vehicle frame.
+void orc18_veh_rhs(int nr, const double* flat, const double* rec, const double* fx, const double* fy, double* acc) {
+    d18::Vehicle<double> s;
+    s.nr = nr != 0;
+    d18::read_params(flat, s.nr, s.vp, s.tp);
+    d18::from_record(s, rec);
+    d18::veh_rhs(s.vs, s.vp, fx, fy);
+    acc[0] = s.vs.udot; acc[1] = s.vs.vdot; acc[2] = s.vs.wxdot; acc[3] = s.vs.wzdot;
+}
+
+}  // extern "C"

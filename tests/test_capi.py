@@ -1,0 +1,127 @@
+"""Host-side checks of the C ABI library (no GPU needed): exports, parameter parsing parity with the
+reference's parser, driver-file loading, argument and ordering errors."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import lfdm_amd
+from lfdm_amd import LfdmError, SolverHalfImplicitGPU, TMeasy, TMeasyNr
+from lfdm_amd._lib import lib
+from cases18 import CASES, paths
+from conftest import DATA, ROOT
+
+HEADER = os.path.join(ROOT, "include", "lfdm_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const char\*|int)\s+(lfdm_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and nothing else is exported under the lfdm_ prefix that the header does not declare
+    out = os.popen(f"nm -D --defined-only {lfdm_amd.LIB_PATH}").read()
+    exported = set(re.findall(r"\b(lfdm_\w+)\b", out))
+    assert exported == set(names), exported ^ set(names)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_params_match_reference_parser(name, golden18):
+    veh, tire, nr, _, _ = paths(name)
+    flat = lfdm_amd.params_from_json(18, veh, tire, tire_type=TMeasyNr if nr else TMeasy)
+    np.testing.assert_array_equal(flat, golden18[f"{name}/params"])
+
+
+def test_record_layout():
+    assert lfdm_amd.record_len(18, TMeasy) == 75
+    assert lfdm_amd.record_len(18, TMeasyNr) == 59
+    f = lfdm_amd.record_fields(18, TMeasy)
+    assert f[4] == "lf.omega" and f[60] == "x" and f[-1] == "current_gr"
+    with pytest.raises(LfdmError):
+        lfdm_amd.record_len(7, TMeasy)
+
+
+def test_driver_file_loader_matches_text():
+    path = os.path.join(DATA, "input", "acc3.txt")
+    rows = lfdm_amd.load_driver_file(path)
+    np.testing.assert_array_equal(rows, np.loadtxt(path, ndmin=2)[:, :4])
+
+
+def test_parse_errors_are_reported():
+    d = os.path.join(DATA, "18dof")
+    # the bundled HMMWV TMeasyNr file spells its class key "VehicleType"; the reference aborts on it
+    with pytest.raises(LfdmError) as e:
+        lfdm_amd.params_from_json(18, f"{d}/HMMWV/vehicle.json", f"{d}/HMMWV/tmeasyNr.json", tire_type=TMeasyNr)
+    assert e.value.name == "LFDM_E_PARSE" and "vehicleType" in str(e.value)
+    with pytest.raises(LfdmError) as e:
+        lfdm_amd.params_from_json(18, f"{d}/nope.json", f"{d}/HMMWV/tmeasy.json")
+    assert e.value.name == "LFDM_E_IO"
+    with pytest.raises(LfdmError) as e:
+        lfdm_amd.load_driver_file(f"{d}/HMMWV/vehicle.json")  # JSON is no driver table
+    assert e.value.name == "LFDM_E_IO"
+
+
+def test_handle_lifecycle_errors(tmp_path):
+    d = os.path.join(DATA, "18dof")
+    veh, tire = f"{d}/HMMWV/vehicle.json", f"{d}/HMMWV/tmeasy.json"
+    drv = os.path.join(DATA, "input", "acc3.txt")
+    s = SolverHalfImplicitGPU(18, 10)
+    s.Construct(veh, tire, 6, drv)
+    with pytest.raises(LfdmError) as e:  # more vehicles than the handle holds
+        s.Construct(veh, tire, 5, drv)
+    assert e.value.name == "LFDM_E_CAPACITY"
+    with pytest.raises(LfdmError) as e:  # tire types cannot be mixed within one solver
+        s.Construct(f"{d}/Sedan/vehicle.json", f"{d}/Sedan/tmeasyNr.json", 2, drv, TMeasyNr)
+    assert e.value.name == "LFDM_E_ARG"
+    s.Construct(veh, tire, 4, drv)
+    rec = s.zero_record()
+    s.Initialize(rec, 10)
+    with pytest.raises(LfdmError) as e:  # Solve before SetEndTime (the reference asserts)
+        s.Solve()
+    assert e.value.name == "LFDM_E_STATE"
+    bad = rec.copy()
+    bad[60] = np.nan
+    s2 = SolverHalfImplicitGPU(18, 2)
+    s2.Construct(veh, tire, 2, drv)
+    with pytest.raises(LfdmError):
+        s2.Initialize(bad, 2)
+    # states are readable before the first launch
+    np.testing.assert_array_equal(s.GetSimState(3), rec)
+    with pytest.raises(LfdmError):
+        s.GetSimState(10)
+    s.SetOutput(str(tmp_path / "out"), 100)
+    with pytest.raises(LfdmError) as e:  # SolveStep with CSV output (the reference exits)
+        s.SolveStep(0.0, 0.0, 0.5, 0.0)
+    assert e.value.name == "LFDM_E_STATE"
+    assert s.GetStep() == 1e-3 and s.GetTireType() == TMeasy
+    s.SetTimeStep(5e-4)
+    s.SetEndTime(3.0)
+    assert s.GetStep() == 5e-4 and s.GetEndTime() == 3.0
+    for bad_call in (lambda: s.SetTimeStep(0.0), lambda: s.SetKernelSimTime(-1), lambda: s.SetThreadsPerBlock(0)):
+        with pytest.raises(LfdmError):
+            bad_call()
+    s.close()
+    s2.close()
+
+
+def test_errors_without_gpu_are_loud():
+    # with no GPU the first device use must fail with LFDM_E_HIP, never fall back to the CPU
+    from conftest import gpu_available
+
+    if gpu_available():
+        pytest.skip("GPU present")
+    d = os.path.join(DATA, "18dof")
+    s = SolverHalfImplicitGPU(18, 1)
+    s.Construct(f"{d}/HMMWV/vehicle.json", f"{d}/HMMWV/tmeasy.json", 1, os.path.join(DATA, "input", "acc3.txt"))
+    s.Initialize(s.zero_record(), 1)
+    s.SetEndTime(1.0)
+    with pytest.raises(LfdmError) as e:
+        s.Solve()
+    assert e.value.name == "LFDM_E_HIP"
