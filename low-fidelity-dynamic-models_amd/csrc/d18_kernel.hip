@@ -78,8 +78,21 @@ __device__ __forceinline__ Ctl controls_at(const Launch18& L, const P18& P, Ctrl
     return c;
 }
 
+// Occupancy target of the integration kernels (waves per SIMD); 0 leaves it to the compiler.
+// The step needs more registers than two waves per SIMD leave (the compiler spills some to
+// scratch), yet two resident waves hide more FP64 latency than the spills cost: 5.0e9 versus
+// 3.8e9 vehicle-steps/s at one wave per SIMD on the 1M-vehicle fleet.
+#ifndef LFDM_WAVES_PER_EU
+#define LFDM_WAVES_PER_EU 2
+#endif
+#if LFDM_WAVES_PER_EU > 0
+#define LFDM_OCCUPANCY __attribute__((amdgpu_waves_per_eu(LFDM_WAVES_PER_EU, LFDM_WAVES_PER_EU)))
+#else
+#define LFDM_OCCUPANCY
+#endif
+
 template <bool NR, int MODE>
-__global__ __launch_bounds__(256) void d18_integrate(Launch18 L) {
+__global__ __launch_bounds__(256) LFDM_OCCUPANCY void d18_integrate(Launch18 L) {
     using R = Rec18<NR>;
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int wave = uniform(slot >> 6);

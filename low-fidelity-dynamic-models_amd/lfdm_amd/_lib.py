@@ -7,7 +7,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liblfdm_amd.so")
+LIB_PATH = os.environ.get("LFDM_AMD_LIB") or os.path.join(_HERE, "liblfdm_amd.so")
 
 LFDM_OK = 0
 LFDM_TIRE_TMEASY = 0

