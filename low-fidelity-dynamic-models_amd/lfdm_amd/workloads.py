@@ -29,8 +29,14 @@ def data_path(*parts):
 HMMWV_VEH = data_path("18dof", "HMMWV", "vehicle.json")
 HMMWV_TIRE = data_path("18dof", "HMMWV", "tmeasy.json")
 
-# Index of fields in the flat 18-DOF parameter vector (see csrc/params.h).
-FLAT18 = {"a": 0, "b": 1, "h": 2, "m": 3, "jz": 4, "jx": 5, "jxz": 6, "maxSteer": 17, "maxBrakeTorque": 19}
+# Index of fields in the flat 11-, 18- and 24-DOF parameter vectors (layouts in csrc/params.h);
+# "gears" is the index of the gear count that opens the variable-length part.
+FLAT18 = {"a": 0, "b": 1, "h": 2, "m": 3, "jz": 4, "jx": 5, "jxz": 6, "maxSteer": 17, "maxBrakeTorque": 19,
+          "gears": 25}
+FLAT11 = {"a": 0, "b": 1, "h": 2, "m": 3, "jz": 4, "maxSteer": 7, "maxBrakeTorque": 9, "gears": 15}
+FLAT24 = {"a": 0, "b": 1, "h": 2, "m": 3, "jz": 4, "jx": 5, "jy": 6, "maxSteer": 13, "maxBrakeTorque": 15,
+          "gears": 21}
+FLAT = {11: FLAT11, 18: FLAT18, 24: FLAT24}
 
 
 def time_grid(nsteps, dt, t0=0.0):
@@ -43,9 +49,9 @@ def time_grid(nsteps, dt, t0=0.0):
     return t
 
 
-def tire_offsets(flat, tire_nr=False):
-    """Index of dfy0Pn / dfy0P2n in a flat 18-DOF vector."""
-    i = 25
+def tire_offsets(flat, tire_nr=False, dof=18):
+    """Index of dfy0Pn / dfy0P2n in a flat parameter vector."""
+    i = FLAT[dof]["gears"]
     ng = int(flat[i])
     i += 1 + ng
     for _ in range(6):
@@ -57,11 +63,13 @@ def tire_offsets(flat, tire_nr=False):
     return base + 23, base + 24
 
 
-def default_overrides(flat, n, tire_nr=False):
-    """(7, n) per-vehicle parameter array holding the group's own values."""
-    i0, i1 = tire_offsets(flat, tire_nr)
-    vals = [flat[FLAT18["m"]], flat[FLAT18["jz"]], flat[FLAT18["jx"]], flat[i0], flat[i1], 1.0,
-            flat[FLAT18["maxBrakeTorque"]]]
+def default_overrides(flat, n, tire_nr=False, dof=18):
+    """(7, n) per-vehicle parameter array holding the group's own values (jx is 0 for the 11-DOF
+    model, which has no roll)."""
+    F = FLAT[dof]
+    i0, i1 = tire_offsets(flat, tire_nr, dof)
+    vals = [flat[F["m"]], flat[F["jz"]], flat[F["jx"]] if "jx" in F else 0.0, flat[i0], flat[i1], 1.0,
+            flat[F["maxBrakeTorque"]]]
     return np.repeat(np.array(vals)[:, None], n, axis=1)
 
 
@@ -78,12 +86,12 @@ def ramp_sine(n, nsteps=10000, dt=1e-3, seed=1, max_steer=0.582178801):
     return ctrl
 
 
-def knot_fleet(n, flat, seed=2, horizon=10.0, max_steer=None, jitter=0.1):
+def knot_fleet(n, flat, seed=2, horizon=10.0, max_steer=None, jitter=0.1, dof=18, tire_nr=False):
     """Knot program and per-vehicle parameters of the knot_fleet workload.
 
     Returns dict(ns, ds, np, dp, knots (ns + 2 np, n), ovr (7, n))."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    max_steer = flat[FLAT18["maxSteer"]] if max_steer is None else max_steer
+    max_steer = flat[FLAT[dof]["maxSteer"]] if max_steer is None else max_steer
     ds, dp = 0.5, 1.0
     ns = int(round(horizon / ds)) + 1
     np_ = int(round(horizon / dp))
@@ -91,7 +99,7 @@ def knot_fleet(n, flat, seed=2, horizon=10.0, max_steer=None, jitter=0.1):
     thr = rng.uniform(0.0, 1.0, (np_, n))
     brk = np.where(rng.random((np_, n)) < 0.1, rng.uniform(0.0, 0.5, (np_, n)), 0.0)
     knots = np.concatenate([steer, thr, brk], axis=0)
-    ovr = default_overrides(flat, n)
+    ovr = default_overrides(flat, n, tire_nr, dof)
     sm = rng.uniform(1 - jitter, 1 + jitter, n)
     sj = rng.uniform(1 - jitter, 1 + jitter, n)
     sc = rng.uniform(1 - jitter, 1 + jitter, n)

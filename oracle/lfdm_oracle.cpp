@@ -10,7 +10,9 @@
 #include <vector>
 
 #include "orc_count.h"
+#include "orc11.h"
 #include "orc18.h"
+#include "orc24.h"
 
 namespace orc {
 thread_local OpCount g_ops;
@@ -63,6 +65,99 @@ struct Controls {
     }
 };
 
+
+// Per-model plumbing for the generic runner below.
+struct M18 {
+    template <class T> using V = d18::Vehicle<T>;
+    static int rec(bool nr) { return d18::record_len(nr); }
+    template <class T> static void params(V<T>& s, const double* flat) { d18::read_params(flat, s.nr, s.vp, s.tp); }
+    template <class T> static void ovr(V<T>& s, const double* o, int n, int i) { apply_overrides_18(s, o, n, i); }
+    template <class T> static void step(V<T>& s, const T& st, const T& th, const T& br, double dt) {
+        d18::step(s, st, th, br, dt);
+    }
+    template <class T> static void get(const V<T>& s, double* o) { d18::to_record(s, o); }
+    template <class T> static void set(V<T>& s, const double* o) { d18::from_record(s, o); }
+};
+struct M11 {
+    template <class T> using V = d11::Vehicle<T>;
+    static int rec(bool nr) { return d11::record_len(nr); }
+    template <class T> static void params(V<T>& s, const double* flat) { d11::read_params(flat, s.nr, s.vp, s.tp); }
+    template <class T> static void ovr(V<T>& s, const double* o, int n, int i) {
+        if (!o) return;
+        s.vp.m = T(o[0 * n + i]);
+        s.vp.jz = T(o[1 * n + i]);
+        s.tp.dfy0Pn = T(o[3 * n + i]);
+        s.tp.dfy0P2n = T(o[4 * n + i]);
+        const double sc = o[5 * n + i];
+        for (auto& e : s.vp.torque) e.y = e.y * sc;
+        s.vp.maxBrakeTorque = T(o[6 * n + i]);
+    }
+    template <class T> static void step(V<T>& s, const T& st, const T& th, const T& br, double dt) {
+        d11::step(s, st, th, br, dt);
+    }
+    template <class T> static void get(const V<T>& s, double* o) { d11::to_record(s, o); }
+    template <class T> static void set(V<T>& s, const double* o) { d11::from_record(s, o); }
+};
+struct M24 {
+    template <class T> using V = d24::Vehicle<T>;
+    static int rec(bool nr) { return d24::record_len(nr); }
+    template <class T> static void params(V<T>& s, const double* flat) {
+        d24::read_params(flat, s.nr, s.vp, s.tp, s.sp);
+    }
+    template <class T> static void ovr(V<T>& s, const double* o, int n, int i) {
+        if (!o) return;
+        s.vp.m = T(o[0 * n + i]);
+        s.vp.jz = T(o[1 * n + i]);
+        s.vp.jx = T(o[2 * n + i]);
+        s.tp.dfy0Pn = T(o[3 * n + i]);
+        s.tp.dfy0P2n = T(o[4 * n + i]);
+        const double sc = o[5 * n + i];
+        for (auto& e : s.vp.torque) e.y = e.y * sc;
+        s.vp.maxBrakeTorque = T(o[6 * n + i]);
+    }
+    template <class T> static void step(V<T>& s, const T& st, const T& th, const T& br, double dt) {
+        d24::step(s, st, th, br, dt);
+    }
+    template <class T> static void get(const V<T>& s, double* o) { d24::to_record(s, o); }
+    template <class T> static void set(V<T>& s, const double* o) { d24::from_record(s, o); }
+};
+
+template <class M>
+int run_model(int nr, const double* flat, const double* ovr, const double* init, int n, double t0, double dt,
+              int nsteps, int mode, const double* ctrl, const int* imeta, const double* dmeta, int every,
+              double* out, double* fin, int threads) {
+    const int rl = M::rec(nr != 0);
+    Controls cs{mode, ctrl, imeta, dmeta, n};
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < n; ++i) {
+        typename M::template V<double> s;
+        s.nr = nr != 0;
+        M::params(s, flat);
+        M::ovr(s, ovr, n, i);
+        if (init) {
+            M::set(s, init + (size_t)i * rl);
+        } else {
+            std::vector<double> z(rl, 0.0);
+            M::set(s, z.data());
+        }
+        double t = t0;
+        int rec = 0;
+        for (int k = 0; k < nsteps; ++k) {
+            double st, th, br;
+            cs.at(i, k, t, st, th, br);
+            M::step(s, st, th, br, dt);
+            t = t + dt;
+            if (out && every > 0 && (k + 1) % every == 0) {
+                M::get(s, out + ((size_t)rec * n + i) * rl);
+                ++rec;
+            }
+        }
+        if (fin) M::get(s, fin + (size_t)i * rl);
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -76,34 +171,36 @@ int orc18_record_len(int nr) { return d18::record_len(nr != 0); }
 int orc18_run(int nr, const double* flat, const double* ovr, const double* init, int n, double t0, double dt,
               int nsteps, int mode, const double* ctrl, const int* imeta, const double* dmeta, int every, double* out,
               double* fin, int threads) {
-    const int rl = d18::record_len(nr != 0);
-    Controls cs{mode, ctrl, imeta, dmeta, n};
-    if (threads > 0) omp_set_num_threads(threads);
-#pragma omp parallel for schedule(dynamic, 1)
+    return run_model<M18>(nr, flat, ovr, init, n, t0, dt, nsteps, mode, ctrl, imeta, dmeta, every, out, fin,
+                          threads);
+}
+
+int orc11_record_len(int nr) { return d11::record_len(nr != 0); }
+int orc11_run(int nr, const double* flat, const double* ovr, const double* init, int n, double t0, double dt,
+              int nsteps, int mode, const double* ctrl, const int* imeta, const double* dmeta, int every, double* out,
+              double* fin, int threads) {
+    return run_model<M11>(nr, flat, ovr, init, n, t0, dt, nsteps, mode, ctrl, imeta, dmeta, every, out, fin,
+                          threads);
+}
+
+int orc24_record_len(int nr) { return d24::record_len(nr != 0); }
+int orc24_run(int nr, const double* flat, const double* ovr, const double* init, int n, double t0, double dt,
+              int nsteps, int mode, const double* ctrl, const int* imeta, const double* dmeta, int every, double* out,
+              double* fin, int threads) {
+    return run_model<M24>(nr, flat, ovr, init, n, t0, dt, nsteps, mode, ctrl, imeta, dmeta, every, out, fin,
+                          threads);
+}
+// initializeTireSus on n records in place (vehicle i uses the per-vehicle parameters of ovr, if any).
+int orc24_init(int nr, const double* flat, const double* ovr, int n, double* recs) {
+    const int rl = d24::record_len(nr != 0);
     for (int i = 0; i < n; ++i) {
-        d18::Vehicle<double> s;
+        d24::Vehicle<double> s;
         s.nr = nr != 0;
-        d18::read_params(flat, s.nr, s.vp, s.tp);
-        apply_overrides_18(s, ovr, n, i);
-        if (init) {
-            d18::from_record(s, init + (size_t)i * rl);
-        } else {
-            std::vector<double> z(rl, 0.0);
-            d18::from_record(s, z.data());
-        }
-        double t = t0;
-        int rec = 0;
-        for (int k = 0; k < nsteps; ++k) {
-            double st, th, br;
-            cs.at(i, k, t, st, th, br);
-            d18::step(s, st, th, br, dt);
-            t = t + dt;
-            if (out && every > 0 && (k + 1) % every == 0) {
-                d18::to_record(s, out + ((size_t)rec * n + i) * rl);
-                ++rec;
-            }
-        }
-        if (fin) d18::to_record(s, fin + (size_t)i * rl);
+        M24::params(s, flat);
+        M24::ovr(s, ovr, n, i);
+        d24::from_record(s, recs + (size_t)i * rl);
+        d24::init_tire_sus(s);
+        d24::to_record(s, recs + (size_t)i * rl);
     }
     return 0;
 }

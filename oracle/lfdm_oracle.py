@@ -1,14 +1,16 @@
 """ORACLE — test infrastructure only.
 
 Python binding of oracle/liblfdm_oracle.so, the CPU restatement of the reference half-implicit
-solvers (wheeled_vehicle_models/18dof/dof18.cpp, dof18_halfImplicit.cpp).  Only tests/,
+solvers (wheeled_vehicle_models/{11,18,24}dof/dof{11,18,24}.cpp and dof*_halfImplicit.cpp).  Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module, and only as the
 checker or the CPU baseline, never as the thing measured.
 
 Pinning (see tests/test_oracle.py): the 18-DOF restatement is bit-identical to the reference
 solver compiled from its own sources (oracle/_ref) on the golden trajectories in tests/golden/
 (HMMWV TMeasy, Sedan TMeasyNr, ART TMeasy with throttleMod, audi RWD; driver files and random
-per-step controls), and checked against the reference's own known-answer values.
+per-step controls), and checked against the reference's own known-answer values.  The 11- and
+24-DOF restatements are pinned the same way (tests/test_oracle1124.py, tests/golden/d11_golden.npz,
+d24_golden.npz).
 """
 import ctypes
 import os
@@ -30,8 +32,13 @@ def lib():
         p, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
         L.orc18_record_len.argtypes = [i]
         L.orc18_record_len.restype = i
-        L.orc18_run.argtypes = [i, p, p, p, i, d, d, i, i, p, p, p, i, p, p, i]
-        L.orc18_run.restype = i
+        for m in ("11", "18", "24"):
+            getattr(L, f"orc{m}_record_len").argtypes = [i]
+            getattr(L, f"orc{m}_record_len").restype = i
+            getattr(L, f"orc{m}_run").argtypes = [i, p, p, p, i, d, d, i, i, p, p, p, i, p, p, i]
+            getattr(L, f"orc{m}_run").restype = i
+        L.orc24_init.argtypes = [i, p, p, i, p]
+        L.orc24_init.restype = i
         L.orc18_count_ops.argtypes = [i, p, p, d, d, i, i, p, p, p, p]
         L.orc18_count_ops.restype = i
         L.orc_tmxy.argtypes = [d, d, d, d, d, d, p, p]
@@ -73,20 +80,37 @@ def _ctrl(mode_args):
 
 
 def run18(flat, controls, n, nsteps, tire_nr=False, ovr=None, init=None, t0=0.0, dt=1e-3, every=0, threads=0):
-    """Integrates n vehicles of one 18-DOF parameter set.
+    return run(18, flat, controls, n, nsteps, tire_nr, ovr, init, t0, dt, every, threads)
+
+
+def record_len(dof, tire_nr=False):
+    return getattr(lib(), f"orc{dof}_record_len")(int(tire_nr))
+
+
+def init24(flat, records, tire_nr=False, ovr=None):
+    """initializeTireSus (dof24.cpp:34-89) applied to records (n, rec), returned as a new array."""
+    recs = np.array(records, dtype=np.float64, order="C", copy=True)
+    flat = np.ascontiguousarray(flat, dtype=np.float64)
+    ovr = None if ovr is None else np.ascontiguousarray(ovr, dtype=np.float64)
+    lib().orc24_init(int(tire_nr), flat.ctypes.data, _ptr(ovr), recs.shape[0], recs.ctypes.data)
+    return recs
+
+
+def run(dof, flat, controls, n, nsteps, tire_nr=False, ovr=None, init=None, t0=0.0, dt=1e-3, every=0, threads=0):
+    """Integrates n vehicles of one parameter set of the 11-, 18- or 24-DOF model.
 
     controls: ("steps", (nsteps, 3, n)) | ("table", rows (m, 4)) | ("knots", ns, ds, np, dp, (ns+2np, n)).
     Returns (final records (n, rec), samples (nsteps // every, n, rec) or None)."""
     L = lib()
-    rl = L.orc18_record_len(int(tire_nr))
+    rl = record_len(dof, tire_nr)
     flat = np.ascontiguousarray(flat, dtype=np.float64)
     mode, ctrl, im, dm = _ctrl(controls)
     ovr = None if ovr is None else np.ascontiguousarray(ovr, dtype=np.float64)
     init = None if init is None else np.ascontiguousarray(np.broadcast_to(init, (n, rl)), dtype=np.float64)
     fin = np.zeros((n, rl))
     out = np.zeros((nsteps // every, n, rl)) if every > 0 else None
-    L.orc18_run(int(tire_nr), _ptr(flat), _ptr(ovr), _ptr(init), n, t0, dt, nsteps, mode, _ptr(ctrl), _ptr(im),
-                _ptr(dm), every, _ptr(out), _ptr(fin), threads)
+    getattr(L, f"orc{dof}_run")(int(tire_nr), _ptr(flat), _ptr(ovr), _ptr(init), n, t0, dt, nsteps, mode,
+                                 _ptr(ctrl), _ptr(im), _ptr(dm), every, _ptr(out), _ptr(fin), threads)
     return fin, out
 
 

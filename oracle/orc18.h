@@ -93,8 +93,8 @@ void read_params(const double* flat, bool nr, VehParam<T>& v, TireParam<T>& t) {
 }
 
 // dof18.cpp:15-42
-template <class T>
-T drive_torque(const VehParam<T>& v, const T& throttle, const T& omega) {
+template <class T, class VP>
+T drive_torque(const VP& v, const T& throttle, const T& omega) {
     if (v.throttleMod) {
         std::vector<MapE<T>> m = v.torque;
         for (size_t i = 0; i < m.size(); ++i) {
@@ -132,8 +132,8 @@ void differential_split(const T& torque, double max_bias, const T& sl, const T& 
     }
 }
 
-template <class T>
-T steer_angle(const VehParam<T>& v, const T& steering) {
+template <class T, class VP>
+T steer_angle(const VP& v, const T& steering) {
     if (v.nonLinearSteer) return map_y(v.steer, steering);
     return steering * v.maxSteer;
 }
@@ -257,25 +257,19 @@ void tire_loads(const Vehicle<T>& s, T* loads) {
     loads[3] = val(l3) > 0. ? l3 : T(0.);
 }
 
-// dof18.cpp:380-539
+// dof18.cpp:380-539.  The core (from the effective rolling radius on) is shared with the 11-DOF
+// model (identical code, dof11.cpp:312-472) and the 24-DOF model (dof24.cpp:827-987), which
+// differ only in how fz and rStat are formed before it.
+// The TMeasy force and deflection-rate computation once fz and rStat are known.
 template <class T>
-void tire_rhs_tmeasy(TireState<T>& ts, const TireParam<T>& tp, const VehParam<T>& vp, const T& steering) {
+void tire_core_tmeasy(TireState<T>& ts, const TireParam<T>& tp, const T& delta, T fz) {
     using std::atan2;
     using std::fabs;
     using std::hypot;
     using std::sqrt;
     using std::tan;
-    T delta = steer_angle(vp, steering);
-    T fz = ts.fz;
     T vsy = ts.vsy;
     T vsx = ts.vsx;
-    if (val(fz) <= 0) {
-        ts.xe = T(0); ts.ye = T(0); ts.omega = T(0); ts.xt = T(0); ts.rStat = tp.r0;
-        ts.fx = T(0); ts.fy = T(0); ts.My = T(0);
-        return;
-    }
-    ts.xt = fz / tp.kt;
-    ts.rStat = tp.r0 - ts.xt;
     T r_eff, rdynco;
     if (val(fz) <= val(tp.fzRdynco)) {
         rdynco = interp_l(fz, tp.rdyncoPn, tp.rdyncoP2n, tp.pn);
@@ -336,25 +330,31 @@ void tire_rhs_tmeasy(TireState<T>& ts, const TireParam<T>& tp, const VehParam<T>
     ts.fy = weighty * fystr + (1. - weighty) * fydyn;
 }
 
-// dof18.cpp:541-673
+template <class T, class VP>
+void tire_rhs_tmeasy(TireState<T>& ts, const TireParam<T>& tp, const VP& vp, const T& steering) {
+    T delta = steer_angle(vp, steering);
+    T fz = ts.fz;
+    if (val(fz) <= 0) {
+        ts.xe = T(0); ts.ye = T(0); ts.omega = T(0); ts.xt = T(0); ts.rStat = tp.r0;
+        ts.fx = T(0); ts.fy = T(0); ts.My = T(0);
+        return;
+    }
+    ts.xt = fz / tp.kt;
+    ts.rStat = tp.r0 - ts.xt;
+    tire_core_tmeasy(ts, tp, delta, fz);
+}
+
+// dof18.cpp:541-673 (shared like the TMeasy tire: dof11.cpp:473-606, dof24.cpp:988-1121)
 template <class T>
-void tire_rhs_tmeasy_nr(TireState<T>& ts, const TireParam<T>& tp, const VehParam<T>& vp, const T& steering) {
+void tire_core_tmeasy_nr(TireState<T>& ts, const TireParam<T>& tp, const T& delta, T fz) {
     using std::atan2;
     using std::fabs;
     using std::hypot;
     using std::sqrt;
     using std::tan;
     using std::tanh;
-    T delta = steer_angle(vp, steering);
-    T fz = ts.fz;
     T vsy = ts.vsy;
     T vsx = ts.vsx;
-    if (val(fz) <= 0) {
-        ts.omega = T(0); ts.xt = T(0); ts.rStat = tp.r0; ts.fx = T(0); ts.fy = T(0); ts.My = T(0);
-        return;
-    }
-    ts.xt = fz / tp.kt;
-    ts.rStat = tp.r0 - ts.xt;
     T r_eff, rdynco;
     if (val(fz) <= val(tp.fzRdynco)) {
         rdynco = interp_l(fz, tp.rdyncoPn, tp.rdyncoP2n, tp.pn);
@@ -413,13 +413,27 @@ void tire_rhs_tmeasy_nr(TireState<T>& ts, const TireParam<T>& tp, const VehParam
     ts.fy = Fy;
 }
 
-// dof18.cpp:675-845 (the Nr overload :847-1017 is the same code)
-template <class T>
-void powertrain_rhs(Vehicle<T>& s, const T& throttle, const T& braking) {
+template <class T, class VP>
+void tire_rhs_tmeasy_nr(TireState<T>& ts, const TireParam<T>& tp, const VP& vp, const T& steering) {
+    T delta = steer_angle(vp, steering);
+    T fz = ts.fz;
+    if (val(fz) <= 0) {
+        ts.omega = T(0); ts.xt = T(0); ts.rStat = tp.r0; ts.fx = T(0); ts.fy = T(0); ts.My = T(0);
+        return;
+    }
+    ts.xt = fz / tp.kt;
+    ts.rStat = tp.r0 - ts.xt;
+    tire_core_tmeasy_nr(ts, tp, delta, fz);
+}
+
+// dof18.cpp:675-845 (the Nr overload :847-1017 is the same code; the 24-DOF model uses it too,
+// dof24.cpp:1338-1509)
+template <class T, class V>
+void powertrain_rhs(V& s, const T& throttle, const T& braking) {
     using std::fabs;
     const auto& v = s.vp;
     auto& st = s.vs;
-    TireState<T>* t = s.tire;
+    auto* t = s.tire;
     T torque_t = T(0);
     const double max_bias = 2;
     if (v.tcbool) {
@@ -533,7 +547,7 @@ void step(Vehicle<T>& s, const T& steering, const T& throttle, const T& braking,
         tire_rhs_tmeasy_nr(s.tire[2], s.tp, s.vp, T(0.0));
         tire_rhs_tmeasy_nr(s.tire[3], s.tp, s.vp, T(0.0));
     }
-    powertrain_rhs(s, throttle, braking);
+    powertrain_rhs<T>(s, throttle, braking);
     tire_to_veh(s, steering);
     T fx[4] = {s.tire[0].fx, s.tire[1].fx, s.tire[2].fx, s.tire[3].fx};
     T fy[4] = {s.tire[0].fy, s.tire[1].fy, s.tire[2].fy, s.tire[3].fy};

@@ -24,6 +24,18 @@ def golden18():
         return {k: z[k] for k in z.files}
 
 
+@pytest.fixture(scope="session")
+def golden11():
+    with np.load(os.path.join(GOLDEN, "d11_golden.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(scope="session")
+def golden24():
+    with np.load(os.path.join(GOLDEN, "d24_golden.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
 def gpu_available():
     try:
         import torch

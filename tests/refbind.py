@@ -1,5 +1,5 @@
-"""ctypes binding of oracle/_ref/libref18.so: the reference CPU half-implicit solver compiled from
-its own sources (wheeled_vehicle_models/18dof/*.cpp) plus the shim oracle/ref_shim18.cpp.
+"""ctypes binding of oracle/_ref/libref{11,18,24}.so: the reference CPU half-implicit solvers compiled
+from their own sources (wheeled_vehicle_models/{11,18,24}dof/*.cpp) plus the shims oracle/ref_shim*.cpp.
 
 Present only where oracle/_ref was built (make -C oracle ref, which needs the reference sources);
 tests that use it skip when it is absent.
@@ -20,44 +20,62 @@ def available(dof=18):
 _libs = {}
 
 
-def lib18():
-    if 18 not in _libs:
-        L = ctypes.CDLL(os.path.join(REF_DIR, "libref18.so"))
+def _lib(dof):
+    if dof not in _libs:
+        L = ctypes.CDLL(os.path.join(REF_DIR, f"libref{dof}.so"))
         p, d, i, s = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_char_p
-        L.ref18_create.restype = p
-        L.ref18_create.argtypes = [s, s, i, s]
-        L.ref18_destroy.argtypes = [p]
-        L.ref18_set_step.argtypes = [p, d]
-        L.ref18_set_param.argtypes = [p, s, d]
-        L.ref18_set_param.restype = i
-        L.ref18_dump_params.argtypes = [p, p]
-        L.ref18_dump_params.restype = i
-        L.ref18_get_record.argtypes = [p, p]
-        L.ref18_set_record.argtypes = [p, p]
-        L.ref18_run.argtypes = [p, d, i, p, i, p]
-        L.ref18_run.restype = d
-        L.ref18_solve.argtypes = [p]
-        L.ref18_record_len.argtypes = [i]
-        L.ref18_record_len.restype = i
-        _libs[18] = L
-    return _libs[18]
+        f = lambda name: getattr(L, f"ref{dof}_{name}")  # noqa: E731
+        f("create").restype = p
+        f("create").argtypes = [s, s, s, i, s] if dof == 24 else [s, s, i, s]
+        f("destroy").argtypes = [p]
+        f("set_step").argtypes = [p, d]
+        f("set_param").argtypes = [p, s, d]
+        f("set_param").restype = i
+        f("dump_params").argtypes = [p, p]
+        f("dump_params").restype = i
+        f("get_record").argtypes = [p, p]
+        f("run").argtypes = [p, d, i, p, i, p]
+        f("run").restype = d
+        f("solve").argtypes = [p]
+        f("record_len").argtypes = [i]
+        f("record_len").restype = i
+        if dof == 18:
+            f("set_record").argtypes = [p, p]
+        if dof == 24:
+            f("initialize").argtypes = [p, d]
+        _libs[dof] = L
+    return _libs[dof]
 
 
-class Ref18:
-    """One reference d18SolverHalfImplicit (constructed from JSON files, zero initial state)."""
+def lib18():
+    return _lib(18)
 
-    def __init__(self, veh, tire, tire_nr=False, driver=None, dt=1e-3):
-        L = lib18()
-        self.L = L
+
+class RefSolver:
+    """One reference d{11,18,24}SolverHalfImplicit constructed from JSON files, zero initial state
+    (24-DOF: Initialize with zero states and forward speed u0, which runs initializeTireSus).
+    overrides: {name: value} set on the parsed parameters before Initialize."""
+
+    def __init__(self, dof, veh, tire, tire_nr=False, driver=None, dt=1e-3, sus=None, u0=0.0, overrides=None):
+        L = _lib(dof)
+        self.L, self.dof = L, dof
+        self._f = lambda name: getattr(L, f"ref{dof}_{name}")  # noqa: E731
         self.nr = bool(tire_nr)
-        self.h = L.ref18_create(os.fsencode(veh), os.fsencode(tire), int(tire_nr),
-                                os.fsencode(driver) if driver else None)
-        L.ref18_set_step(self.h, dt)
-        self.rl = L.ref18_record_len(int(tire_nr))
+        drv = os.fsencode(driver) if driver else None
+        if dof == 24:
+            self.h = self._f("create")(os.fsencode(veh), os.fsencode(tire), os.fsencode(sus), int(tire_nr), drv)
+        else:
+            self.h = self._f("create")(os.fsencode(veh), os.fsencode(tire), int(tire_nr), drv)
+        self._f("set_step")(self.h, dt)
+        for k, v in (overrides or {}).items():
+            self.set_param(k, v)
+        if dof == 24:
+            self._f("initialize")(self.h, float(u0))
+        self.rl = self._f("record_len")(int(tire_nr))
 
     def close(self):
         if self.h:
-            self.L.ref18_destroy(self.h)
+            self._f("destroy")(self.h)
             self.h = None
 
     def __del__(self):
@@ -65,28 +83,33 @@ class Ref18:
 
     def params(self):
         buf = np.zeros(8192)
-        n = self.L.ref18_dump_params(self.h, buf.ctypes.data)
+        n = self._f("dump_params")(self.h, buf.ctypes.data)
         return buf[:n].copy()
 
     def set_param(self, name, value):
-        assert self.L.ref18_set_param(self.h, name.encode(), float(value)) == 0, name
+        assert self._f("set_param")(self.h, name.encode(), float(value)) == 0, name
 
     def record(self):
         r = np.zeros(self.rl)
-        self.L.ref18_get_record(self.h, r.ctypes.data)
+        self._f("get_record")(self.h, r.ctypes.data)
         return r
 
     def set_record(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
-        self.L.ref18_set_record(self.h, r.ctypes.data)
+        self._f("set_record")(self.h, r.ctypes.data)
 
     def run(self, nsteps, ctrl=None, every=0, t0=0.0):
         """ctrl: (nsteps, 3) steering, throttle, braking or None for the driver file."""
         c = None if ctrl is None else np.ascontiguousarray(ctrl, dtype=np.float64)
         out = np.zeros((nsteps // every, self.rl)) if every > 0 else None
-        t = self.L.ref18_run(self.h, t0, nsteps, None if c is None else c.ctypes.data, every,
-                             None if out is None else out.ctypes.data)
+        t = self._f("run")(self.h, t0, nsteps, None if c is None else c.ctypes.data, every,
+                           None if out is None else out.ctypes.data)
         return t, out
 
     def solve(self):
-        self.L.ref18_solve(self.h)
+        self._f("solve")(self.h)
+
+
+class Ref18(RefSolver):
+    def __init__(self, veh, tire, tire_nr=False, driver=None, dt=1e-3):
+        super().__init__(18, veh, tire, tire_nr, driver, dt)
