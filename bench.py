@@ -1,5 +1,8 @@
 """Throughput benchmark: 18-DOF FP64 vehicle-steps/s on 1,000,000 vehicles (one MI355X).
 
+`--dof 11` / `--dof 24` run the same workload on the 11- and 24-DOF models (24-DOF vehicles start
+at U(0, 20) m/s with rolling wheels); the default line is the 18-DOF headline.
+
 Workload (knot_fleet): HMMWV vehicle + TMeasy tires, dt = 1e-3 s, mass / yaw inertia / cornering
 stiffness jittered x U(0.9, 1.1) per vehicle, throttle piecewise constant U(0, 1) per 1 s,
 steering U(-0.3, 0.3) rad knots every 0.5 s (linear), brake U(0, 0.5) on a seeded 10 % of the
@@ -7,8 +10,8 @@ steering U(-0.3, 0.3) rad knots every 0.5 s (linear), brake U(0, 0.5) on a seede
 steps) in one launch, with every input already resident in HBM.
 
 Prints one JSON line; `roofline` uses the algorithmic FP64 FLOPs per vehicle-step counted by the
-oracle's counting scalar on the same workload (see DESIGN.md for the counting rule) and the
-device time of the integration launch from HIP events on the solver's stream.  `cpu_baseline`
+oracle's counting scalar on the same workload (+ - * / sqrt count 1, sin cos tan atan2 tanh
+count 20, hypot counts 4) and the device time of the integration launch from HIP events on the solver's stream.  `cpu_baseline`
 times the oracle (the CPU restatement, bit-identical to the reference solver) with OpenMP over
 vehicles on a bounded sample of the same workload.
 """
@@ -27,40 +30,54 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector / matrix peak
 HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 
 
-def flops_per_step(flat, nsteps=10000, nveh=4, seed=2):
+def flops_per_step(dof, flat, nsteps=10000, nveh=4, seed=2):
     """Algorithmic FLOPs per vehicle-step: + - * / sqrt count 1, sin/cos/tan/atan2/tanh count 20,
     hypot counts 4 (oracle counting scalar over the knot_fleet workload)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import lfdm_oracle as O
     from lfdm_amd import workloads as W
 
-    fl = W.knot_fleet(nveh, flat, seed=seed)
+    fl = W.knot_fleet(nveh, flat, seed=seed, dof=dof)
+    init = initial_records(O, dof, flat, nveh, fl["ovr"])
     basic = trans = 0
     for i in range(nveh):
         knots = fl["knots"][:, i:i + 1]
-        c = O.count_ops18(flat, ("knots", fl["ns"], fl["ds"], fl["np"], fl["dp"], knots), nsteps)
+        c = O.count_ops(dof, flat, ("knots", fl["ns"], fl["ds"], fl["np"], fl["dp"], knots), nsteps,
+                        init=None if init is None else init[i])
         basic += c["add"] + c["mul"] + c["div"] + c["sqrt"]
         trans += 20 * (c["sin"] + c["cos"] + c["tan"] + c["atan2"] + c["tanh"]) + 4 * c["hypot"]
     steps = nsteps * nveh
     return (basic + trans) / steps, basic / steps
 
 
-def cpu_baseline(flat, n_sample, nsteps, threads):
+def initial_records(O, dof, flat, n, ovr, seed=7):
+    """24-DOF fleets start at U(0, 20) m/s with rolling wheels (workloads.initial_speed_records)
+    after initializeTireSus; the other models start at rest (None)."""
+    if dof != 24:
+        return None
+    from lfdm_amd import workloads as W
+
+    return O.init24(flat, W.initial_speed_records(n, flat, dof, seed=seed), False, ovr)
+
+
+def cpu_baseline(dof, flat, n_sample, nsteps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import lfdm_oracle as O
     from lfdm_amd import workloads as W
 
-    fl = W.knot_fleet(n_sample, flat, seed=2)
+    fl = W.knot_fleet(n_sample, flat, seed=2, dof=dof)
     ctrl = ("knots", fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
+    init = initial_records(O, dof, flat, n_sample, fl["ovr"])
     t0 = time.perf_counter()
-    O.run18(flat, ctrl, n_sample, nsteps, ovr=fl["ovr"], threads=threads)
+    O.run(dof, flat, ctrl, n_sample, nsteps, ovr=fl["ovr"], init=init, threads=threads)
     dt = time.perf_counter() - t0
     # single-core rate on a small subset
     n1 = 64
-    fl1 = W.knot_fleet(n1, flat, seed=3)
+    fl1 = W.knot_fleet(n1, flat, seed=3, dof=dof)
+    init1 = initial_records(O, dof, flat, n1, fl1["ovr"])
     t1 = time.perf_counter()
-    O.run18(flat, ("knots", fl1["ns"], fl1["ds"], fl1["np"], fl1["dp"], fl1["knots"]), n1, nsteps, ovr=fl1["ovr"],
-            threads=1)
+    O.run(dof, flat, ("knots", fl1["ns"], fl1["ds"], fl1["np"], fl1["dp"], fl1["knots"]), n1, nsteps, ovr=fl1["ovr"],
+          init=init1, threads=1)
     d1 = time.perf_counter() - t1
     return n_sample * nsteps / dt, n1 * nsteps / d1, dt
 
@@ -80,28 +97,49 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--dof", type=int, default=18, choices=(11, 18, 24),
+                    help="model; the headline line is 18-DOF (11/24 use the same generators, 24-DOF vehicles start "
+                         "at U(0, 20) m/s)")
     ap.add_argument("--vehicles", type=int, default=1_000_000)
     ap.add_argument("--horizon-steps", type=int, default=10000)
     ap.add_argument("--tpb", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=32768)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (reads 1 GiB, writes 1 GiB), the "
+                         "reference kernel that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 8, "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; vehicles shard across GPUs with no "
-                                   "communication (see DESIGN.md), run with --gpus 1"}))
+                                   "communication between them, run with --gpus 1"}))
         sys.exit(2)
 
     import lfdm_amd
     from lfdm_amd import SolverHalfImplicitGPU, TMeasy
     from lfdm_amd import workloads as W
 
-    n, H = args.vehicles, args.horizon_steps
-    flat = lfdm_amd.params_from_json(18, W.HMMWV_VEH, W.HMMWV_TIRE)
-    fl = W.knot_fleet(n, flat, seed=2)
-    s = SolverHalfImplicitGPU(18, n)
+    n, H, dof = args.vehicles, args.horizon_steps, args.dof
+    d = os.path.join(W.DATA, f"{dof}dof", "HMMWV")
+    flat = lfdm_amd.params_from_json(dof, f"{d}/vehicle.json", f"{d}/tmeasy.json",
+                                     f"{d}/suspension.json" if dof == 24 else None)
+    fl = W.knot_fleet(n, flat, seed=2, dof=dof)
+    s = SolverHalfImplicitGPU(dof, n)
     s.construct_flat(flat, n, None, TMeasy)
-    s.Initialize(s.zero_record(), n)
     s.set_overrides(fl["ovr"])
+    if dof == 24:
+        s.initialize_batch(W.initial_speed_records(n, flat, dof, seed=7))
+    else:
+        s.Initialize(s.zero_record(), n)
     s.set_controls_knots(fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
     s.SetThreadsPerBlock(args.tpb)
     del fl
@@ -119,26 +157,30 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     value = n * H * args.steps / wall
     kernel_ms = float(np.mean(launch_ms))
-    fl_total, fl_basic = flops_per_step(flat)
+    fl_total, fl_basic = flops_per_step(dof, flat)
     achieved_tf = fl_total * n * H / (kernel_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
-                "kernel": "d18_integrate<TMeasy, knots>", "kernel_ms": kernel_ms,
+                "kernel": f"integrate<M{dof}, TMeasy, knots>", "kernel_ms": kernel_ms,
                 "flop_per_vehicle_step": fl_total, "basic_flop_per_vehicle_step": fl_basic}
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        rate, rate1, secs = cpu_baseline(flat, args.cpu_sample, H, threads)
+        rate, rate1, secs = cpu_baseline(dof, flat, args.cpu_sample, H, threads)
         cpu = {"value": rate, "unit": "vehicle-steps/s", "cores": threads, "kind": "port",
                "sample": f"knot_fleet, {args.cpu_sample} vehicles x {H} steps ({secs:.1f} s), "
                          f"oracle (bit-identical CPU restatement), OpenMP over vehicles",
                "single_core": rate1, "cpu": cpu_model()}
-    line = {"metric": "18-DOF FP64 vehicle-steps/s at 1M vehicles", "value": value, "unit": "vehicle-steps/s",
+    metric = ("18-DOF FP64 vehicle-steps/s at 1M vehicles; adjoint/forward cost ratio" if dof == 18 else
+              f"{dof}-DOF FP64 vehicle-steps/s")
+    line = {"metric": metric, "value": value,
+            "unit": "vehicle-steps/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic",
-            "config": {"workload": "knot_fleet: 18-DOF HMMWV TMeasy, 1,000,000 vehicles, 10 s at dt=1e-3, "
-                                   "+-10% mass/Izz/cornering stiffness, knot controls",
+            "config": {"workload": f"knot_fleet: {dof}-DOF HMMWV TMeasy, {n:,} vehicles, {H * 1e-3:g} s at dt=1e-3, "
+                                   "+-10% mass/Izz/cornering stiffness, knot controls" +
+                                   (", start at U(0,20) m/s" if dof == 24 else ""),
                        "vehicles": n, "horizon_steps": H, "dt": 1e-3, "threads_per_block": args.tpb},
             "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(line))

@@ -6,11 +6,23 @@
  * negative LFDM_E_*) and lfdm_last_error() describes the last failure on the calling thread.
  * Plain pointers and sizes only: host buffers unless a name says "device".
  *
+ * The 11- and 24-DOF classes are d11SolverHalfImplicitGPU (11dof-gpu/dof11_halfImplicit_gpu.cuh:16)
+ * and d24SolverHalfImplicitGPU (24dof-gpu/dof24_halfImplicit_gpu.cuh:15); their entry points are
+ * the ones cited below for the 18-DOF class at the corresponding lines of those headers.
+ *
  * Records: a vehicle's full state is exchanged as a flat array of doubles in a fixed order
  * (lfdm_record_len gives its length):
  *   18-DOF, per tire LF RF LR RR, TMeasy  : xe ye xedot yedot omega dOmega xt rStat fx fy fz vsx vsy My engTor
  *                                 TMeasyNr: omega dOmega xt rStat fx fy fz vsx vsy My engTor
  *           then the vehicle              : x y u v psi wz phi wx udot vdot wxdot wzdot crankOmega dOmega_crank gear
+ *   11-DOF, per tire F R (fields as 18-DOF), then x y u v psi wz udot vdot wzdot crankOmega dOmega_crank gear
+ *   24-DOF, per tire LF RF LR RR, TMeasy  : xe ye xedot yedot omega dOmega xt dxt rStat fx fy fz fxg fyg fzg
+ *                                           vgx vgy vgz vsx vsy vsz My engTor (TMeasyNr without xe..yedot)
+ *           per strut LF RF LR RR         : xs xsi dxs ls lsi us vs ws dus dvs dws uu vu wu duu dvu dwu fxs fys
+ *                                           fzs fzd mx my mz
+ *           then the vehicle              : x y z u v w psi wz phi wx theta wy udot vdot wdot wxdot wydot wzdot
+ *                                           dtheta dphi dpsi crankOmega dOmega_crank gear
+ * The gear index is stored as a double.
  */
 #ifndef LFDM_AMD_H
 #define LFDM_AMD_H
@@ -67,8 +79,15 @@ int lfdm_construct(lfdm_solver* h, const char* veh_json, const char* tire_json, 
 int lfdm_construct_flat(lfdm_solver* h, const double* flat, int len, int tire_type, unsigned n,
                         const double* rows, int nrows);
 
-/* Initialize(states..., n), .cuh:172-195: appends n vehicles with the given full record. */
+/* Initialize(states..., n), .cuh:172-195: appends n vehicles with the given full record.  For the
+ * 24-DOF model each appended record then goes through initializeTireSus as in
+ * dof24_halfImplicit_gpu.cu:320-333 (static tire loads and compressions, strut preload and lengths,
+ * unsprung heave velocity), using the vehicle's own mass from lfdm_set_overrides when that was
+ * called before Initialize.  Records given to lfdm_set_states are taken as they are. */
 int lfdm_initialize(lfdm_solver* h, const double* record, unsigned n);
+/* Bulk form of Initialize (no reference counterpart; the reference appends one shared state per
+ * call): appends n vehicles with their own records records[n][record_len], same 24-DOF rule. */
+int lfdm_initialize_batch(lfdm_solver* h, const double* records, unsigned n);
 
 /* Setters / getters, .cuh:93-129 */
 int lfdm_set_end_time(lfdm_solver* h, double tend);

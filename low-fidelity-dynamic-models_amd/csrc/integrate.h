@@ -1,0 +1,397 @@
+// Half-implicit integration kernels of the 11-, 18- and 24-DOF models (gfx950), shared by the
+// per-model kernel files d11_kernel.hip, d18_kernel.hip and d24_kernel.hip.
+//
+// One vehicle per lane, 64 vehicles per wavefront, all belonging to one parameter set.  A launch
+// advances every vehicle `nsteps` steps with its carried state in registers; state is read from
+// and written to the SoA record array [field][slot] once per launch.  Output rows are written
+// decimated as [sample][channel][vehicle] like the reference's device response (9 channels for
+// 11-DOF, 13 for 18-DOF, 15 for 24-DOF).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "d11_device.h"
+#include "d18_device.h"
+#include "d24_device.h"
+#include "kernels.h"
+
+namespace lfdm {
+
+using namespace dev;
+
+namespace kern {
+
+struct Ctl {
+    double s, th, b;
+};
+
+template <int MODE>
+struct CtrlState {
+    int kcur = -1;
+    double s0 = 0, s1 = 0, thr = 0, brk = 0;
+};
+
+// Per-vehicle control buffers are indexed by user vehicle index `uv` with stride n_user.
+template <int MODE>
+__device__ __forceinline__ Ctl controls_at(const Launch& L, const PSet& P, CtrlState<MODE>& cs, int uv, int k,
+                                           double t) {
+    Ctl c;
+    if (MODE == kCtrlConst) {
+        c.s = L.c_s; c.th = L.c_t; c.b = L.c_b;
+    } else if (MODE == kCtrlTable) {
+        driver_input(t, L.drv + P.drv_off, P.drv_rows, c.s, c.th, c.b);
+    } else if (MODE == kCtrlSteps) {
+        const size_t N = (size_t)L.n_user;
+        const double* p = L.steps + (size_t)k * 3 * N + uv;
+        c.s = p[0];
+        c.th = p[N];
+        c.b = p[2 * N];
+    } else {  // kCtrlKnots
+        const size_t N = (size_t)L.n_user;
+        int ks = (int)floor(t / L.ds);
+        ks = ks < 0 ? 0 : (ks > L.ns - 2 ? L.ns - 2 : ks);
+        int kp = (int)floor(t / L.dp);
+        kp = kp < 0 ? 0 : (kp > L.np - 1 ? L.np - 1 : kp);
+        const int key = ks * 4096 + kp;  // wave-uniform: t is the same on every lane
+        if (key != cs.kcur) {
+            cs.kcur = key;
+            cs.s0 = L.knots[(size_t)ks * N + uv];
+            cs.s1 = L.knots[(size_t)(ks + 1) * N + uv];
+            cs.thr = L.knots[(size_t)(L.ns + kp) * N + uv];
+            cs.brk = L.knots[(size_t)(L.ns + L.np + kp) * N + uv];
+        }
+        const double tk = ks * L.ds;
+        double tbar = (t - tk) / L.ds;
+        if (tbar > 1.0) tbar = 1.0;
+        c.s = cs.s0 + tbar * (cs.s1 - cs.s0);
+        c.th = cs.thr;
+        c.b = cs.brk;
+    }
+    return c;
+}
+
+// ---- model traits: record layout, state load/store, output channels ----
+
+// 18-DOF record: per tire LF RF LR RR (TMeasy: xe ye xedot yedot) omega dOmega xt rStat fx fy fz vsx
+// vsy My engTor; vehicle x y u v psi wz phi wx udot vdot wxdot wzdot crankOmega dOmega_crank gear.
+struct M18 {
+    using S = S18;
+    using X = X18;
+    using K = K18;
+    static constexpr int kCh = 13;
+    template <bool NR>
+    struct R {
+        static constexpr int kTire = NR ? 11 : 15, kVeh = 4 * kTire, om = NR ? 0 : 4;
+        static __device__ __forceinline__ int f(int i, int o) { return i * kTire + om + o; }  // o: omega = 0 ...
+    };
+    template <bool NR>
+    static __device__ __forceinline__ void load(const double* st, size_t S, int slot, S18& s) {
+        using Rr = R<NR>;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!NR) {
+                s.xe[i] = st[(i * Rr::kTire + 0) * S + slot];
+                s.ye[i] = st[(i * Rr::kTire + 1) * S + slot];
+                s.xedot[i] = st[(i * Rr::kTire + 2) * S + slot];
+                s.yedot[i] = st[(i * Rr::kTire + 3) * S + slot];
+            } else {
+                s.xe[i] = s.ye[i] = s.xedot[i] = s.yedot[i] = 0.0;
+            }
+            s.omega[i] = st[Rr::f(i, 0) * S + slot];
+        }
+        const double* v = st + (size_t)Rr::kVeh * S + slot;
+        s.x = v[0 * S]; s.y = v[1 * S]; s.u = v[2 * S]; s.v = v[3 * S]; s.psi = v[4 * S]; s.wz = v[5 * S];
+        s.phi = v[6 * S]; s.wx = v[7 * S]; s.udot = v[8 * S]; s.vdot = v[9 * S]; s.crank = v[12 * S];
+        s.gear = (int)v[14 * S];
+    }
+    static __device__ __forceinline__ void kinit(const PSet& P, const double* o, size_t S, K18& k) {
+        k18_init(P, o[0], o[S], o[2 * S], o[3 * S], o[4 * S], o[5 * S], o[6 * S], k);
+    }
+    template <bool NR, bool FULL>
+    static __device__ __forceinline__ void step(const PSet& P, const Derived& u, const K18& k, S18& s, const Ctl& c,
+                                                double dt, X18* x) {
+        step18<NR, FULL>(P, u, k, s, c.s, c.th, c.b, dt, x);
+    }
+    static __device__ __forceinline__ void emit(double* o, size_t nu, double t, const S18& s) {
+        o[0 * nu] = t; o[1 * nu] = s.x; o[2 * nu] = s.y; o[3 * nu] = s.u; o[4 * nu] = s.v;
+        o[5 * nu] = s.phi; o[6 * nu] = s.psi; o[7 * nu] = s.wx; o[8 * nu] = s.wz;
+        o[9 * nu] = s.omega[0]; o[10 * nu] = s.omega[1]; o[11 * nu] = s.omega[2]; o[12 * nu] = s.omega[3];
+    }
+    template <bool NR>
+    static __device__ __forceinline__ void store(double* w, size_t S, int slot, const PSet& P, const S18& s,
+                                                 const X18& x) {
+        using Rr = R<NR>;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!NR) {
+                w[(i * Rr::kTire + 0) * S + slot] = s.xe[i];
+                w[(i * Rr::kTire + 1) * S + slot] = s.ye[i];
+                w[(i * Rr::kTire + 2) * S + slot] = s.xedot[i];
+                w[(i * Rr::kTire + 3) * S + slot] = s.yedot[i];
+            }
+            const double f[11] = {s.omega[i], x.dOmega[i], x.xt[i], x.rStat[i], x.fx[i],    x.fy[i],
+                                  x.fz[i],    x.vsx[i],    x.vsy[i], x.My[i],   x.engTor[i]};
+#pragma unroll
+            for (int j = 0; j < 11; ++j) w[Rr::f(i, j) * S + slot] = f[j];
+        }
+        double* v = w + (size_t)Rr::kVeh * S + slot;
+        v[0 * S] = s.x; v[1 * S] = s.y; v[2 * S] = s.u; v[3 * S] = s.v; v[4 * S] = s.psi; v[5 * S] = s.wz;
+        v[6 * S] = s.phi; v[7 * S] = s.wx; v[8 * S] = s.udot; v[9 * S] = s.vdot; v[10 * S] = x.wxdot;
+        v[11 * S] = x.wzdot; v[12 * S] = s.crank;
+        if (P.tcbool) v[13 * S] = x.dcrank;
+        v[14 * S] = (double)s.gear;
+    }
+};
+
+// 11-DOF record: per tire F R (TMeasy: xe ye xedot yedot) omega dOmega xt rStat fx fy fz vsx vsy My
+// engTor; vehicle x y u v psi wz udot vdot wzdot crankOmega dOmega_crank gear.
+struct M11 {
+    using S = S11;
+    using X = X11;
+    using K = K11;
+    static constexpr int kCh = 9;
+    template <bool NR>
+    struct R {
+        static constexpr int kTire = NR ? 11 : 15, kVeh = 2 * kTire, om = NR ? 0 : 4;
+        static __device__ __forceinline__ int f(int i, int o) { return i * kTire + om + o; }
+    };
+    template <bool NR>
+    static __device__ __forceinline__ void load(const double* st, size_t S, int slot, S11& s) {
+        using Rr = R<NR>;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            if (!NR) {
+                s.xe[i] = st[(i * Rr::kTire + 0) * S + slot];
+                s.ye[i] = st[(i * Rr::kTire + 1) * S + slot];
+                s.xedot[i] = st[(i * Rr::kTire + 2) * S + slot];
+                s.yedot[i] = st[(i * Rr::kTire + 3) * S + slot];
+            } else {
+                s.xe[i] = s.ye[i] = s.xedot[i] = s.yedot[i] = 0.0;
+            }
+            s.omega[i] = st[Rr::f(i, 0) * S + slot];
+        }
+        const double* v = st + (size_t)Rr::kVeh * S + slot;
+        s.x = v[0 * S]; s.y = v[1 * S]; s.u = v[2 * S]; s.v = v[3 * S]; s.psi = v[4 * S]; s.wz = v[5 * S];
+        s.udot = v[6 * S]; s.crank = v[9 * S];
+        s.gear = (int)v[11 * S];
+    }
+    static __device__ __forceinline__ void kinit(const PSet& P, const double* o, size_t S, K11& k) {
+        k11_init(P, o[0], o[S], o[3 * S], o[4 * S], o[5 * S], o[6 * S], k);
+    }
+    template <bool NR, bool FULL>
+    static __device__ __forceinline__ void step(const PSet& P, const Derived& u, const K11& k, S11& s, const Ctl& c,
+                                                double dt, X11* x) {
+        step11<NR, FULL>(P, u, k, s, c.s, c.th, c.b, dt, x);
+    }
+    static __device__ __forceinline__ void emit(double* o, size_t nu, double t, const S11& s) {
+        o[0 * nu] = t; o[1 * nu] = s.x; o[2 * nu] = s.y; o[3 * nu] = s.u; o[4 * nu] = s.v;
+        o[5 * nu] = s.psi; o[6 * nu] = s.wz; o[7 * nu] = s.omega[0]; o[8 * nu] = s.omega[1];
+    }
+    template <bool NR>
+    static __device__ __forceinline__ void store(double* w, size_t S, int slot, const PSet& P, const S11& s,
+                                                 const X11& x) {
+        using Rr = R<NR>;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            if (!NR) {
+                w[(i * Rr::kTire + 0) * S + slot] = s.xe[i];
+                w[(i * Rr::kTire + 1) * S + slot] = s.ye[i];
+                w[(i * Rr::kTire + 2) * S + slot] = s.xedot[i];
+                w[(i * Rr::kTire + 3) * S + slot] = s.yedot[i];
+            }
+            const double f[11] = {s.omega[i], x.dOmega[i], x.xt[i], x.rStat[i], x.fx[i],    x.fy[i],
+                                  x.fz[i],    x.vsx[i],    x.vsy[i], x.My[i],   x.engTor[i]};
+#pragma unroll
+            for (int j = 0; j < 11; ++j) w[Rr::f(i, j) * S + slot] = f[j];
+        }
+        double* v = w + (size_t)Rr::kVeh * S + slot;
+        v[0 * S] = s.x; v[1 * S] = s.y; v[2 * S] = s.u; v[3 * S] = s.v; v[4 * S] = s.psi; v[5 * S] = s.wz;
+        v[6 * S] = s.udot; v[7 * S] = x.vdot; v[8 * S] = x.wzdot; v[9 * S] = s.crank;
+        if (P.tcbool) v[10 * S] = x.dcrank;
+        v[11 * S] = (double)s.gear;
+    }
+};
+
+// 24-DOF record: per tire LF RF LR RR (TMeasy: xe ye xedot yedot) omega dOmega xt dxt rStat fx fy fz
+// fxg fyg fzg vgx vgy vgz vsx vsy vsz My engTor; per strut xs xsi dxs ls lsi us vs ws dus dvs dws uu
+// vu wu duu dvu dwu fxs fys fzs fzd mx my mz; vehicle x y z u v w psi wz phi wx theta wy udot vdot
+// wdot wxdot wydot wzdot dtheta dphi dpsi crankOmega dOmega_crank gear.
+struct M24 {
+    using S = S24;
+    using X = X24;
+    using K = K24;
+    static constexpr int kCh = 15;
+    template <bool NR>
+    struct R {
+        static constexpr int kTire = NR ? 19 : 23, kSus = 4 * kTire, kVeh = kSus + 4 * 24, om = NR ? 0 : 4;
+        static __device__ __forceinline__ int f(int i, int o) { return i * kTire + om + o; }
+        static __device__ __forceinline__ int q(int i, int o) { return kSus + i * 24 + o; }
+    };
+    // tire fields after omega: dOmega 1, xt 2, dxt 3, rStat 4, fx 5, fy 6, fz 7, fxg 8, fyg 9, fzg 10,
+    // vgx 11, vgy 12, vgz 13, vsx 14, vsy 15, vsz 16, My 17, engTor 18
+    template <bool NR>
+    static __device__ __forceinline__ void load(const double* st, size_t S, int slot, S24& s) {
+        using Rr = R<NR>;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!NR) {
+                s.xe[i] = st[(i * Rr::kTire + 0) * S + slot];
+                s.ye[i] = st[(i * Rr::kTire + 1) * S + slot];
+                s.xedot[i] = st[(i * Rr::kTire + 2) * S + slot];
+                s.yedot[i] = st[(i * Rr::kTire + 3) * S + slot];
+            } else {
+                s.xe[i] = s.ye[i] = s.xedot[i] = s.yedot[i] = 0.0;
+            }
+            s.omega[i] = st[Rr::f(i, 0) * S + slot];
+            s.xt[i] = st[Rr::f(i, 2) * S + slot];
+            s.rStat[i] = st[Rr::f(i, 4) * S + slot];
+            s.vsz[i] = st[Rr::f(i, 16) * S + slot];
+            s.xs[i] = st[Rr::q(i, 0) * S + slot];
+            s.xsi[i] = st[Rr::q(i, 1) * S + slot];
+            s.dxs[i] = st[Rr::q(i, 2) * S + slot];
+            s.lsi[i] = st[Rr::q(i, 4) * S + slot];
+            s.wu[i] = st[Rr::q(i, 13) * S + slot];
+            s.fzd[i] = st[Rr::q(i, 20) * S + slot];
+        }
+        const double* v = st + (size_t)Rr::kVeh * S + slot;
+        s.x = v[0 * S]; s.y = v[1 * S]; s.u = v[3 * S]; s.v = v[4 * S]; s.w = v[5 * S]; s.psi = v[6 * S];
+        s.wz = v[7 * S]; s.phi = v[8 * S]; s.wx = v[9 * S]; s.theta = v[10 * S]; s.wy = v[11 * S];
+        s.udot = v[12 * S]; s.vdot = v[13 * S]; s.wdot = v[14 * S]; s.wxdot = v[15 * S]; s.wydot = v[16 * S];
+        s.wzdot = v[17 * S]; s.crank = v[21 * S];
+        s.gear = (int)v[23 * S];
+    }
+    static __device__ __forceinline__ void kinit(const PSet& P, const double* o, size_t S, K24& k) {
+        k24_init(P, o[0], o[S], o[2 * S], o[3 * S], o[4 * S], o[5 * S], o[6 * S], k);
+    }
+    template <bool NR, bool FULL>
+    static __device__ __forceinline__ void step(const PSet& P, const Derived& u, const K24& k, S24& s, const Ctl& c,
+                                                double dt, X24* x) {
+        step24<NR, FULL>(P, u, k, s, c.s, c.th, c.b, dt, x);
+    }
+    static __device__ __forceinline__ void emit(double* o, size_t nu, double t, const S24& s) {
+        o[0 * nu] = t; o[1 * nu] = s.x; o[2 * nu] = s.y; o[3 * nu] = s.u; o[4 * nu] = s.v;
+        o[5 * nu] = s.phi; o[6 * nu] = s.psi; o[7 * nu] = s.theta; o[8 * nu] = s.wx; o[9 * nu] = s.wz;
+        o[10 * nu] = s.wy; o[11 * nu] = s.omega[0]; o[12 * nu] = s.omega[1]; o[13 * nu] = s.omega[2];
+        o[14 * nu] = s.omega[3];
+    }
+    template <bool NR>
+    static __device__ __forceinline__ void store(double* w, size_t S, int slot, const PSet& P, const S24& s,
+                                                 const X24& x) {
+        using Rr = R<NR>;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!NR) {
+                w[(i * Rr::kTire + 0) * S + slot] = s.xe[i];
+                w[(i * Rr::kTire + 1) * S + slot] = s.ye[i];
+                w[(i * Rr::kTire + 2) * S + slot] = s.xedot[i];
+                w[(i * Rr::kTire + 3) * S + slot] = s.yedot[i];
+            }
+            // omega .. vgy (vgz, vsz are never written by the model), vsx vsy My engTor
+            const double f[13] = {s.omega[i], x.dOmega[i], s.xt[i],   x.dxt[i],  s.rStat[i], x.fx[i], x.fy[i],
+                                  x.fz[i],    x.fxg[i],    x.fyg[i],  x.fzg[i],  x.vgx[i],   x.vgy[i]};
+#pragma unroll
+            for (int j = 0; j < 13; ++j) w[Rr::f(i, j) * S + slot] = f[j];
+            w[Rr::f(i, 14) * S + slot] = x.vsx[i];
+            w[Rr::f(i, 15) * S + slot] = x.vsy[i];
+            w[Rr::f(i, 17) * S + slot] = x.My[i];
+            w[Rr::f(i, 18) * S + slot] = x.engTor[i];
+            // strut: xs . dxs ls . us vs ws dus dvs . uu vu wu duu dvu dwu fxs fys fzs fzd mx my mz
+            w[Rr::q(i, 0) * S + slot] = s.xs[i];
+            w[Rr::q(i, 2) * S + slot] = s.dxs[i];
+            w[Rr::q(i, 3) * S + slot] = x.ls[i];
+            const double g[5] = {x.us[i], x.vs[i], x.ws[i], x.dus[i], x.dvs[i]};
+#pragma unroll
+            for (int j = 0; j < 5; ++j) w[Rr::q(i, 5 + j) * S + slot] = g[j];
+            const double h[13] = {x.uu[i],  x.vu[i],  s.wu[i],  x.duu[i], x.dvu[i], x.dwu[i], x.fxs[i],
+                                  x.fys[i], x.fzs[i], s.fzd[i], x.mx[i],  x.my[i],  0.0};
+#pragma unroll
+            for (int j = 0; j < 13; ++j) w[Rr::q(i, 11 + j) * S + slot] = h[j];
+        }
+        double* v = w + (size_t)Rr::kVeh * S + slot;
+        v[0 * S] = s.x; v[1 * S] = s.y; v[3 * S] = s.u; v[4 * S] = s.v; v[5 * S] = s.w; v[6 * S] = s.psi;
+        v[7 * S] = s.wz; v[8 * S] = s.phi; v[9 * S] = s.wx; v[10 * S] = s.theta; v[11 * S] = s.wy;
+        v[12 * S] = s.udot; v[13 * S] = s.vdot; v[14 * S] = s.wdot; v[15 * S] = s.wxdot; v[16 * S] = s.wydot;
+        v[17 * S] = s.wzdot; v[18 * S] = x.dtheta; v[19 * S] = x.dphi; v[20 * S] = x.dpsi; v[21 * S] = s.crank;
+        if (P.tcbool) v[22 * S] = x.dcrank;
+        v[23 * S] = (double)s.gear;
+    }
+};
+
+// Occupancy target of the integration kernels (waves per SIMD); 0 leaves it to the compiler.
+// The 18-DOF step needs more registers than two waves per SIMD leave (the compiler spills some to
+// scratch), yet two resident waves hide more FP64 latency than the spills cost: 5.0e9 versus
+// 3.8e9 vehicle-steps/s at one wave per SIMD on the 1M-vehicle fleet.
+#ifndef LFDM_WAVES_PER_EU
+#define LFDM_WAVES_PER_EU 2
+#endif
+#if LFDM_WAVES_PER_EU > 0
+#define LFDM_OCCUPANCY __attribute__((amdgpu_waves_per_eu(LFDM_WAVES_PER_EU, LFDM_WAVES_PER_EU)))
+#else
+#define LFDM_OCCUPANCY
+#endif
+
+template <class M, bool NR, int MODE>
+__global__ __launch_bounds__(256) LFDM_OCCUPANCY void integrate(Launch L) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const int wave = uniform(slot >> 6);
+    if (wave * kWave >= L.S) return;
+    const int set = uniform(L.wave_set[wave]);
+    const PSet& P = L.sets[set];
+    const size_t S = (size_t)L.S;
+
+    typename M::S s;
+    M::template load<NR>(L.state, S, slot, s);
+    const Derived& u = P.d;
+    typename M::K k;
+    M::kinit(P, L.ovr + slot, S, k);
+    const int user = L.slot_user[slot];
+    const int uv = user >= 0 ? user : 0;  // padding lanes read vehicle 0's controls
+    CtrlState<MODE> cs;
+    double t = L.t0;
+    const double dt = L.dt;
+    int oi = 0;
+    int next_out = L.n_out > 0 ? L.out_steps[0] : -1;
+    auto emit = [&](int done) {
+        if (done != next_out) return;
+        if (user >= 0) M::emit(L.out + (size_t)oi * M::kCh * L.n_user + user, (size_t)L.n_user, t, s);
+        ++oi;
+        next_out = oi < L.n_out ? L.out_steps[oi] : -1;
+    };
+    // all steps but the last keep only the carried state ...
+    const int last = L.nsteps - 1;
+    for (int kk = 0; kk < last; ++kk) {
+        const Ctl c = controls_at<MODE>(L, P, cs, uv, kk, t);
+        M::template step<NR, false>(P, u, k, s, c, dt, nullptr);
+        t = t + dt;
+        emit(kk + 1);
+    }
+    // ... the last one also returns the right-hand-side quantities the full record holds
+    typename M::X x;
+    {
+        const Ctl c = controls_at<MODE>(L, P, cs, uv, last, t);
+        M::template step<NR, true>(P, u, k, s, c, dt, &x);
+        t = t + dt;
+        emit(last + 1);
+    }
+    M::template store<NR>(L.state, S, slot, P, s, x);
+}
+
+template <class M, bool NR>
+hipError_t launch_mode(const Launch& L, int block, hipStream_t stream) {
+    const int grid = (L.S + block - 1) / block;
+    switch (L.mode) {
+        case kCtrlTable: hipLaunchKernelGGL((integrate<M, NR, kCtrlTable>), grid, block, 0, stream, L); break;
+        case kCtrlConst: hipLaunchKernelGGL((integrate<M, NR, kCtrlConst>), grid, block, 0, stream, L); break;
+        case kCtrlKnots: hipLaunchKernelGGL((integrate<M, NR, kCtrlKnots>), grid, block, 0, stream, L); break;
+        case kCtrlSteps: hipLaunchKernelGGL((integrate<M, NR, kCtrlSteps>), grid, block, 0, stream, L); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class M>
+hipError_t launch_model(const Launch& L, int block, hipStream_t stream) {
+    return L.nr ? launch_mode<M, true>(L, block, stream) : launch_mode<M, false>(L, block, stream);
+}
+
+}  // namespace kern
+}  // namespace lfdm

@@ -1,9 +1,10 @@
 // Launch records and host-side entry points of the HIP kernels.
 //
-// Canonical 18-DOF record (one per vehicle; SoA on the device as [field][slot]):
-//   per tire LF, RF, LR, RR  TMeasy:   xe ye xedot yedot omega dOmega xt rStat fx fy fz vsx vsy My engTor (15)
-//                            TMeasyNr: omega dOmega xt rStat fx fy fz vsx vsy My engTor             (11)
-//   vehicle:                 x y u v psi wz phi wx udot vdot wxdot wzdot crankOmega dOmega_crank gear  (15)
+// Records (one per vehicle; SoA on the device as [field][slot]); the full field lists are in
+// integrate.h:
+//   11-DOF  2 tires x 15 (TMeasy) / 11 (TMeasyNr) + 12 vehicle fields       = 42 / 34
+//   18-DOF  4 tires x 15 / 11 + 15 vehicle fields                           = 75 / 59
+//   24-DOF  4 tires x 23 / 19 + 4 struts x 24 + 24 vehicle fields           = 212 / 196
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -11,8 +12,8 @@
 
 namespace lfdm {
 
-struct Launch18 {
-    const P18* sets;       // parameter sets
+struct Launch {
+    const PSet* sets;       // parameter sets
     const int* wave_set;   // set index of every 64-slot wavefront
     const int* slot_user;  // user vehicle index of every slot, -1 for padding
     double* state;         // [record field][S]
@@ -21,7 +22,7 @@ struct Launch18 {
     const double* knots;   // [ns + 2 np][n_user] (kCtrlKnots)
     const double* steps;   // [nsteps][3][n_user] for this launch (kCtrlSteps)
     const int* out_steps;  // step counts after which an output row is written, ascending
-    double* out;           // [n_out][13][n_user]
+    double* out;           // [n_out][channels][n_user]
     double t0, dt;
     double c_s, c_t, c_b;  // kCtrlConst
     double ds, dp;         // knot spacings (kCtrlKnots)
@@ -29,7 +30,11 @@ struct Launch18 {
     int nsteps, n_out, S, n_user, mode, nr;
 };
 
-hipError_t launch_d18(const Launch18& L, int block, hipStream_t stream);
+// Integration launches of the 11-, 18- and 24-DOF models (one kernel per tire model and control
+// source inside each).
+hipError_t launch_d11(const Launch& L, int block, hipStream_t stream);
+hipError_t launch_d18(const Launch& L, int block, hipStream_t stream);
+hipError_t launch_d24(const Launch& L, int block, hipStream_t stream);
 
 // Gathers full records of `n` slots from SoA state into out[n][rec].
 hipError_t launch_gather(const double* state, int S, int rec, const int* slots, int n, double* out,

@@ -34,25 +34,29 @@ struct TireP {
     double vcoulomb, frblend_begin, frblend_end, bearingCapacity, li, p_li, p_use;
 };
 
-// Parameter-only subexpressions of one 18-DOF set, evaluated once on the host with the same
-// operands and operation order as the reference evaluates them every step (so the same bits).
-struct D18 {
+// Parameter-only subexpressions of one set, evaluated once on the host with the same operands and
+// operation order as the reference evaluates them every step (so the same bits).  The first row is
+// used by the 18-DOF chassis only.
+struct Derived {
     double two_ab, E2c, Uc, A1, A1sq, jxz2, kro, bro, inv_jw, inv_ci;
     double qx1, qx2, qfxm1, qfxm2, qfym1, qfym2, qfxs1, qfxs2, qfys1, qfys2;  // InterpQ pairs (2w1 - w2/2, w1 - w2/2)
     double lsxm, lsym, lsxs, lsys, lrd;                                       // InterpL slopes (w2 - w1)
     double delta_rear;  // steering angle of the rear tires (steering input 0)
 };
 
-// 18-DOF vehicle parameters (dof18.h:489-799) plus the tire set and the group's driver table.
-struct P18 {
-    double a, b, h, m, jz, jx, jxz, cf, cr, muf, mur, hrcf, hrcr, krof, kror, brof, bror;
+// Vehicle parameters of one set (11-DOF dof11.h:496-620, 18-DOF dof18.h:489-799, 24-DOF
+// dof24.h:580-764 plus SuspensionParam :766-780) with the tire set and the group's driver table.
+// One layout serves the three models; a model reads only its own fields (11-DOF: a b h m jz muf
+// mur; 24-DOF adds jx jy cf cr hrcf hrcr ks bs; 18-DOF: jx jxz cf cr hrcf hrcr krof kror brof bror).
+struct PSet {
+    double a, b, h, m, jz, jx, jy, jxz, cf, cr, muf, mur, hrcf, hrcr, krof, kror, brof, bror, ks, bs;
     double maxSteer, crankInertia, maxBrakeTorque;
-    int nonLinearSteer, tcbool, throttleMod, driveType, whichWheels, ngears, tire_nr, pad0;
+    int nonLinearSteer, tcbool, throttleMod, driveType, whichWheels, ngears, tire_nr, dof;
     double gears[kMaxGears];
     MapXY shift[kMaxGears];
     Map steer, torque, losses, cfmap, trmap;
     TireP tire;
-    D18 d;
+    Derived d;
     int32_t drv_off, drv_rows;  // rows of this group's driver table in the table pool
 };
 

@@ -150,12 +150,17 @@ const char* kCurveKeys[20] = {"dfx0Pn", "dfx0P2n", "fxmPn", "fxmP2n", "fxsPn", "
                               "sxmP2n", "sxsPn",   "sxsP2n", "dfy0Pn", "dfy0P2n", "fymPn", "fymP2n",
                               "fysPn",  "fysP2n",  "symPn", "symP2n", "sysPn",  "sysP2n"};
 
-// Low-level TMeasyNr keys (dof18.cpp:1393-1459); mu, vcoulomb and blending keep their defaults.
-void nr_low_level(Req& r, NrTire& t) {
-    t.jw = r.num("jw"); t.rr = r.num("rr"); t.r0 = r.num("r0"); t.pn = r.num("pn"); t.pnmax = r.num("pnmax");
-    t.kt = r.num("kt"); t.rdyncoPn = r.num("rdyncoPn"); t.rdyncoP2n = r.num("rdyncoP2n");
-    t.fzRdynco = r.num("fzRdynco"); t.rdyncoCrit = r.num("rdyncoCrit");
-    for (int i = 0; i < 20; ++i) t.c[i] = r.num(kCurveKeys[i]);
+// Which curve keys the 11-DOF loaders double: forces and slopes (the single-track model's axle
+// tire stands for the two tires of an axle, dof11.cpp:1036-1086), not the slip points.
+const bool kCurveAxle[20] = {1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0};
+
+// Low-level TMeasyNr keys (dof18.cpp:1393-1459, dof11.cpp:1208-1240 with the axle factor s2 = 2);
+// mu, vcoulomb and blending keep their defaults.
+void nr_low_level(Req& r, NrTire& t, double s2) {
+    t.jw = r.num("jw") * s2; t.rr = r.num("rr") * s2; t.r0 = r.num("r0"); t.pn = r.num("pn") * s2;
+    t.pnmax = r.num("pnmax") * s2; t.kt = r.num("kt") * s2; t.rdyncoPn = r.num("rdyncoPn");
+    t.rdyncoP2n = r.num("rdyncoP2n"); t.fzRdynco = r.num("fzRdynco") * s2; t.rdyncoCrit = r.num("rdyncoCrit") * s2;
+    for (int i = 0; i < 20; ++i) t.c[i] = r.num(kCurveKeys[i]) * (kCurveAxle[i] ? s2 : 1.0);
 }
 
 // Optional inflation pressures (dof18.cpp:1326-1340): both must be positive, else 1/1.
@@ -167,18 +172,42 @@ void pressures(const JValue& d, double& p_li, double& p_use) {
     if (!(p_use > 0.0 && p_li > 0.0)) p_li = p_use = 1.0;
 }
 
+// setSuspensionParamsJSON (dof24.cpp:2192-2216): strut stiffness and damping.
+bool read_suspension(const std::string& sus, std::vector<double>& flat, std::string& err) {
+    if (sus.empty()) {
+        err = "the 24-DOF model needs a suspension JSON file";
+        return false;
+    }
+    JValue d;
+    if (!json_load_file(sus, d, err)) return false;
+    Req r{d, sus, err};
+    const double ks = r.num("ks"), bs = r.num("bs");
+    if (!r.ok) return false;
+    flat.push_back(ks);
+    flat.push_back(bs);
+    return true;
+}
+
 }  // namespace
 
-bool params18_from_json(const std::string& veh, const std::string& tire, int tire_nr, std::vector<double>& flat,
-                        std::string& err) {
+bool params_from_json(int dof, const std::string& veh, const std::string& tire, const std::string& sus, int tire_nr,
+                      std::vector<double>& flat, std::string& err) {
     flat.clear();
+    if (dof != 11 && dof != 18 && dof != 24) {
+        err = "unsupported model: " + std::to_string(dof) + "-DOF";
+        return false;
+    }
     JValue d;
     if (!json_load_file(veh, d, err)) return false;
     Req r{d, veh, err};
-    // setVehParamsJSON, dof18.cpp:1085-1215
-    const char* scal[] = {"a", "b", "h", "m", "jz", "jx", "jxz", "cf", "cr", "muf", "mur", "hrcf", "hrcr",
-                          "krof", "kror", "brof", "bror"};
-    for (const char* k : scal) flat.push_back(r.num(k));
+    // setVehParamsJSON: dof11.cpp:913-1032, dof18.cpp:1085-1215, dof24.cpp:2217-2343
+    static const char* const scal11[] = {"a", "b", "h", "m", "jz", "muf", "mur"};
+    static const char* const scal18[] = {"a",   "b",   "h",    "m",    "jz",   "jx",   "jxz",  "cf",   "cr",
+                                         "muf", "mur", "hrcf", "hrcr", "krof", "kror", "brof", "bror"};
+    static const char* const scal24[] = {"a", "b", "h", "m", "jz", "jx", "jy", "cf", "cr", "muf", "mur", "hrcf", "hrcr"};
+    const char* const* scal = dof == 11 ? scal11 : (dof == 18 ? scal18 : scal24);
+    const int nscal = dof == 11 ? 7 : (dof == 18 ? 17 : 13);
+    for (int i = 0; i < nscal; ++i) flat.push_back(r.num(scal[i]));
     bool nls = r.flag("nonLinearSteer");
     flat.push_back(r.num("maxSteer"));  // read unconditionally (dof18.cpp:1138)
     bool tc = r.flag("tcBool");
@@ -234,25 +263,28 @@ bool params18_from_json(const std::string& veh, const std::string& tire, int tir
     JValue t;
     if (!json_load_file(tire, t, err)) return false;
     Req q{t, tire, err};
+    const double s2 = dof == 11 ? 2.0 : 1.0;  // axle factor of the 11-DOF loaders
     if (!tire_nr) {
-        // setTireParamsJSON(TMeasyParam&), dof18.cpp:1218-1272; mu is not read (default 0.8)
+        // setTireParamsJSON(TMeasyParam&), dof18.cpp:1218-1272 (dof11.cpp:1036-1086 doubles the
+        // stiffnesses, damping, inertia, loads and forces); mu is not read (default 0.8)
         double v[35];
         const char* keys[] = {"jw", "rr", nullptr, "r0", "pn", "pnmax", "cx", "cy", "kt", "dx", "dy", "rdyncoPn",
                               "rdyncoP2n", "fzRdynco", "rdyncoCrit"};
-        for (int i = 0; i < 15; ++i) v[i] = keys[i] ? q.num(keys[i]) : 0.8;
-        for (int i = 0; i < 20; ++i) v[15 + i] = q.num(kCurveKeys[i]);
+        const bool axle[15] = {1, 1, 0, 0, 1, 1, 1, 1, 1, 1, 1, 0, 0, 1, 1};
+        for (int i = 0; i < 15; ++i) v[i] = keys[i] ? q.num(keys[i]) * (axle[i] ? s2 : 1.0) : 0.8;
+        for (int i = 0; i < 20; ++i) v[15 + i] = q.num(kCurveKeys[i]) * (kCurveAxle[i] ? s2 : 1.0);
         if (!q.ok) return false;
         tire_init(v[4], v[11], v[12], v[13], v[14]);
         flat.insert(flat.end(), v, v + 35);
-        return true;
+        return dof == 24 ? read_suspension(sus, flat, err) : true;
     }
     // setTireParamsJSON(TMeasyNrParam&), dof18.cpp:1275-1461
     NrTire n;
     const JValue* hl = t.find("highLevelParams");
     if (hl && hl->kind == JValue::Bool && hl->b) {
-        n.jw = q.num("jw"); n.rr = q.num("rr"); n.mu = q.num("mu"); n.r0 = q.num("r0");
+        n.jw = q.num("jw") * s2; n.rr = q.num("rr") * s2; n.mu = q.num("mu") * s2; n.r0 = q.num("r0");
         n.rdyncoPn = q.num("rdyncoPn"); n.rdyncoP2n = q.num("rdyncoP2n");
-        n.fzRdynco = q.num("fzRdynco"); n.rdyncoCrit = q.num("rdyncoCrit");
+        n.fzRdynco = q.num("fzRdynco") * s2; n.rdyncoCrit = q.num("rdyncoCrit") * s2;
         const JValue* vc = t.find("vCoulomb");
         const JValue* fb = t.find("frblendBegin");
         const JValue* fe = t.find("frblendEnd");
@@ -280,8 +312,10 @@ bool params18_from_json(const std::string& veh, const std::string& tire, int tir
         if (!q.ok) return false;
         const TireClass& k = vt->str == "Truck" ? kTruck80 : kPassCar70;
         guess_tire(k, load, width, (n.r0 - rimRadius) / width, 2 * rimRadius, p_li, p_use, n);
+        // the 11-DOF guessers double the forces and slopes (dof11.cpp:1328-1380)
+        for (int i = 0; i < 20; ++i) n.c[i] = n.c[i] * (kCurveAxle[i] ? s2 : 1.0);
     } else {
-        nr_low_level(q, n);
+        nr_low_level(q, n, s2);
     }
     if (!q.ok) return false;
     tire_init(n.pn, n.rdyncoPn, n.rdyncoP2n, n.fzRdynco, n.rdyncoCrit);
@@ -291,7 +325,7 @@ bool params18_from_json(const std::string& veh, const std::string& tire, int tir
     flat.insert(flat.end(), n.c, n.c + 20);
     const double tail[] = {n.vcoulomb, n.frblend_begin, n.frblend_end, n.bearingCapacity, n.li, n.p_li, n.p_use};
     flat.insert(flat.end(), tail, tail + 7);
-    return true;
+    return dof == 24 ? read_suspension(sus, flat, err) : true;
 }
 
 namespace {
@@ -339,9 +373,9 @@ double host_map_y(const Map& m, double x) {
 // Parameter-only subexpressions, written exactly as the reference writes them inside
 // computeVehRHS (dof18.cpp:1019-1078), computeTireRHS / InterpQ / InterpL (utils.h:93-100) and
 // computePowertrainRHS (dof18.cpp:799-844).
-void derive18(P18& p) {
+void derive18(PSet& p) {
     const TireP& t = p.tire;
-    D18& u = p.d;
+    Derived& u = p.d;
     u.two_ab = 2. * (p.a + p.b);
     u.E2c = -p.muf * p.a + p.mur * p.b;
     u.Uc = -p.mur * p.b + p.muf * p.a;
@@ -367,12 +401,24 @@ void derive18(P18& p) {
 
 }  // namespace
 
-bool p18_from_flat(const double* flat, int len, int tire_nr, P18& o, std::string& err) {
-    o = P18{};
+bool pset_from_flat(int dof, const double* flat, int len, int tire_nr, PSet& o, std::string& err) {
+    o = PSet{};
+    if (dof != 11 && dof != 18 && dof != 24) {
+        err = "unsupported model: " + std::to_string(dof) + "-DOF";
+        return false;
+    }
+    o.dof = dof;
     Rd r{flat, len};
-    double* s[] = {&o.a, &o.b, &o.h, &o.m, &o.jz, &o.jx, &o.jxz, &o.cf, &o.cr, &o.muf, &o.mur, &o.hrcf, &o.hrcr,
-                   &o.krof, &o.kror, &o.brof, &o.bror, &o.maxSteer, &o.crankInertia, &o.maxBrakeTorque};
-    for (double* p : s) *p = r.next();
+    double* s11[] = {&o.a, &o.b, &o.h, &o.m, &o.jz, &o.muf, &o.mur};
+    double* s18[] = {&o.a,   &o.b,    &o.h,    &o.m,    &o.jz,   &o.jx,   &o.jxz,  &o.cf,  &o.cr,
+                     &o.muf, &o.mur,  &o.hrcf, &o.hrcr, &o.krof, &o.kror, &o.brof, &o.bror};
+    double* s24[] = {&o.a, &o.b, &o.h, &o.m, &o.jz, &o.jx, &o.jy, &o.cf, &o.cr, &o.muf, &o.mur, &o.hrcf, &o.hrcr};
+    double** s = dof == 11 ? s11 : (dof == 18 ? s18 : s24);
+    const int ns = dof == 11 ? 7 : (dof == 18 ? 17 : 13);
+    for (int i = 0; i < ns; ++i) *s[i] = r.next();
+    o.maxSteer = r.next();
+    o.crankInertia = r.next();
+    o.maxBrakeTorque = r.next();
     o.nonLinearSteer = r.next() != 0;
     o.tcbool = r.next() != 0;
     o.throttleMod = r.next() != 0;
@@ -409,6 +455,10 @@ bool p18_from_flat(const double* flat, int len, int tire_nr, P18& o, std::string
                         &t.vcoulomb, &t.frblend_begin, &t.frblend_end, &t.bearingCapacity, &t.li, &t.p_li, &t.p_use};
         for (double* p : ts) *p = r.next();
     }
+    if (dof == 24) {
+        o.ks = r.next();
+        o.bs = r.next();
+    }
     if (!r.ok) { err = "flat parameter vector too short"; return false; }
     if (r.i != len) { err = "flat parameter vector has " + std::to_string(len - r.i) + " trailing values"; return false; }
     derive18(o);
@@ -435,6 +485,31 @@ bool load_driver_file(const std::string& path, std::vector<DrvRow>& rows, std::s
         return false;
     }
     return true;
+}
+
+void init_tire_sus24(const PSet& p, double m, int tire_nr, double* rec) {
+    const double g = 9.81;
+    const int tl = tire_nr ? 19 : 23;         // tire block length
+    const int o_xt = tire_nr ? 2 : 6, o_fz = tire_nr ? 7 : 11;
+    double* sus = rec + 4 * tl;               // strut blocks: xs xsi dxs ls lsi ... wu at 13
+    const double w = rec[4 * tl + 4 * 24 + 5];  // vehicle heave velocity
+    double fz[4];
+    fz[0] = fz[1] = ((m * g * p.b) / (2 * (p.a + p.b)) + p.muf * g);
+    fz[2] = fz[3] = ((m * g * p.a) / (2 * (p.a + p.b)) + p.mur * g);
+    double xsi[4];
+    xsi[0] = xsi[1] = (m * g * p.b) / (2 * (p.a + p.b) * p.ks);
+    xsi[2] = xsi[3] = (m * g * p.a) / (2 * (p.a + p.b) * p.ks);
+    for (int i = 0; i < 4; ++i) {
+        const double xt = fz[i] / p.tire.kt;
+        rec[i * tl + o_fz] = fz[i];
+        rec[i * tl + o_xt] = xt;
+        double* q = sus + 24 * i;
+        q[0] = xsi[i];                   // xs
+        q[1] = xsi[i];                   // xsi
+        q[4] = p.h - (p.tire.r0 - xt);   // lsi
+        q[3] = q[4];                     // ls
+        q[13] = w;                       // wu
+    }
 }
 
 }  // namespace lfdm

@@ -1,8 +1,12 @@
 // Setup-time parameter handling: JSON files -> flat parameter vectors -> device records.
 //
-// Flat 18-DOF vector (the exchange format of the C ABI and of the fixtures):
-//   [0..19]  a b h m jz jx jxz cf cr muf mur hrcf hrcr krof kror brof bror maxSteer crankInertia maxBrakeTorque
-//   [20..24] nonLinearSteer tcbool throttleMod driveType whichWheels   (0/1)
+// Flat parameter vectors (the exchange format of the C ABI and of the fixtures), per model:
+//   11-DOF [0..9]   a b h m jz muf mur maxSteer crankInertia maxBrakeTorque
+//   18-DOF [0..19]  a b h m jz jx jxz cf cr muf mur hrcf hrcr krof kror brof bror maxSteer crankInertia
+//                   maxBrakeTorque
+//   24-DOF [0..15]  a b h m jz jx jy cf cr muf mur hrcf hrcr maxSteer crankInertia maxBrakeTorque
+// then, for every model:
+//   nonLinearSteer tcbool throttleMod driveType whichWheels   (0/1)
 //   ngears, gearRatios[ngears]
 //   six maps, each as n, x0, y0, x1, y1, ...: shift (x = downshift, y = upshift, rad/s), steer, torque
 //   (rad/s -> N m), losses, capacity factor, torque ratio
@@ -12,7 +16,9 @@
 //      or 40 TMeasyNr scalars jw rr mu r0 width rim_radius pn pnmax kt rdyncoPn rdyncoP2n fzRdynco rdyncoCrit
 //                            dfx0Pn ... sysP2n (as above) vcoulomb frblend_begin frblend_end
 //                            bearingCapacity li p_li p_use
-// Tire values are stored after tireInit (fzRdynco, rdyncoCrit recomputed).
+//   24-DOF only: suspension ks bs
+// Tire values are stored after tireInit (fzRdynco, rdyncoCrit recomputed) and, for the 11-DOF
+// model, with the loaders' axle factor of 2 applied (its single-track tire stands for an axle).
 #pragma once
 #include <string>
 #include <vector>
@@ -21,9 +27,14 @@
 
 namespace lfdm {
 
-bool params18_from_json(const std::string& veh, const std::string& tire, int tire_nr, std::vector<double>& flat,
-                        std::string& err);
-bool p18_from_flat(const double* flat, int len, int tire_nr, P18& out, std::string& err);
+// Reads the reference's JSON files of one model (dof 11, 18 or 24; `sus` is used by 24 only).
+bool params_from_json(int dof, const std::string& veh, const std::string& tire, const std::string& sus, int tire_nr,
+                      std::vector<double>& flat, std::string& err);
+bool pset_from_flat(int dof, const double* flat, int len, int tire_nr, PSet& out, std::string& err);
+
+// initializeTireSus (dof24.cpp:34-89) on one 24-DOF record, with the vehicle's own mass (which
+// may differ from the set's through the per-vehicle parameters).
+void init_tire_sus24(const PSet& p, double m, int tire_nr, double* rec);
 
 // Driver table file: whitespace-separated "time steering throttle braking" rows, read until the
 // first line that does not parse (utils.cpp:12-31).

@@ -1,7 +1,7 @@
 // Host side of the C ABI: solver handles, groups, device layout, launches and CSV output.
 //
-// Mirrors d18SolverHalfImplicitGPU (wheeled_vehicle_models/18dof-gpu/dof18_halfImplicit_gpu.cu)
-// call for call.  The differences are in where things live: every Construct group becomes one
+// Mirrors d11/d18/d24SolverHalfImplicitGPU (wheeled_vehicle_models/{11,18,24}dof-gpu/
+// dof*_halfImplicit_gpu.cu) call for call.  The differences are in where things live: every Construct group becomes one
 // parameter record shared by its vehicles (not one copy per vehicle), groups are padded to whole
 // 64-wide wavefronts on the device, the state is SoA, and the time base is carried across
 // launches exactly as the reference CPU solver accumulates it (t += dt every step).
@@ -40,14 +40,26 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 int record_len(int dof, int tire_nr) {
+    if (dof == 11) return tire_nr ? 2 * 11 + 12 : 2 * 15 + 12;
     if (dof == 18) return tire_nr ? 4 * 11 + 15 : 4 * 15 + 15;
+    if (dof == 24) return tire_nr ? 4 * 19 + 4 * 24 + 24 : 4 * 23 + 4 * 24 + 24;
     return -1;
 }
+
+// Output channels of the device response (dof11_halfImplicit_gpu.cu:378, dof18 :372,
+// dof24 :430) and their CSV column names (Write, dof11 :561-579, dof18 :583-596, dof24 :608-640).
+int out_channels(int dof) { return dof == 11 ? 9 : (dof == 18 ? 13 : 15); }
+
+const char* const kCols11[] = {"time", "x", "y", "vx", "vy", "yaw", "yaw_rate", "wf", "wr"};
+const char* const kCols18[] = {"time", "x", "y", "vx", "vy", "roll", "yaw", "roll_rate", "yaw_rate",
+                               "wlf", "wrf", "wlr", "wrr"};
+const char* const kCols24[] = {"time", "x",        "y",        "vx",         "vy",  "roll", "yaw", "pitch",
+                               "roll_rate", "yaw_rate", "pitch_rate", "wlf", "wrf",  "wlr", "wrr"};
 
 struct Group {
     std::vector<double> flat;
     int tire_nr = 0;
-    P18 p18;
+    PSet pset;
     std::vector<DrvRow> rows;
     unsigned n = 0;
     unsigned user_base = 0;
@@ -99,7 +111,7 @@ struct lfdm_solver {
     std::vector<int> user_slot;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    P18* d_sets = nullptr;
+    PSet* d_sets = nullptr;
     int* d_wave_set = nullptr;
     int* d_slot_user = nullptr;
     int* d_user_slot = nullptr;
@@ -174,7 +186,7 @@ int ensure_device(lfdm_solver* h) {
     h->S = S;
     std::vector<int> slot_user(S, -1), wave_set(S / kWave, 0);
     h->user_slot.assign(h->total, 0);
-    std::vector<P18> sets(h->groups.size());
+    std::vector<PSet> sets(h->groups.size());
     std::vector<DrvRow> pool;
     for (size_t g = 0; g < h->groups.size(); ++g) {
         const Group& G = h->groups[g];
@@ -184,7 +196,7 @@ int ensure_device(lfdm_solver* h) {
         }
         const int waves = (int)((G.n + kWave - 1) / kWave);
         for (int w = 0; w < waves; ++w) wave_set[slot_base[g] / kWave + w] = (int)g;
-        sets[g] = G.p18;
+        sets[g] = G.pset;
         sets[g].drv_off = (int)pool.size();
         sets[g].drv_rows = (int)G.rows.size();
         pool.insert(pool.end(), G.rows.begin(), G.rows.end());
@@ -203,21 +215,21 @@ int ensure_device(lfdm_solver* h) {
     // per-vehicle parameters: group values unless overridden
     std::vector<double> ovr((size_t)kOverrides * S, 0.0);
     for (int s = 0; s < S; ++s) {
-        const P18& p = sets[wave_set[s / kWave]];
+        const PSet& p = sets[wave_set[s / kWave]];
         const double dflt[kOverrides] = {p.m, p.jz, p.jx, p.tire.dfy0Pn, p.tire.dfy0P2n, 1.0, p.maxBrakeTorque};
         for (int f = 0; f < kOverrides; ++f) ovr[(size_t)f * S + s] = dflt[f];
         const int u = slot_user[s];
         if (u >= 0 && !h->ovr_host.empty())
             for (int f = 0; f < kOverrides; ++f) ovr[(size_t)f * S + s] = h->ovr_host[(size_t)f * h->total + u];
     }
-    HIPCHK(hipMalloc((void**)&h->d_sets, sets.size() * sizeof(P18)));
+    HIPCHK(hipMalloc((void**)&h->d_sets, sets.size() * sizeof(PSet)));
     HIPCHK(hipMalloc((void**)&h->d_wave_set, wave_set.size() * sizeof(int)));
     HIPCHK(hipMalloc((void**)&h->d_slot_user, (size_t)S * sizeof(int)));
     HIPCHK(hipMalloc((void**)&h->d_user_slot, (size_t)h->total * sizeof(int)));
     HIPCHK(hipMalloc((void**)&h->d_state, st.size() * sizeof(double)));
     HIPCHK(hipMalloc((void**)&h->d_ovr, ovr.size() * sizeof(double)));
     HIPCHK(hipMalloc((void**)&h->d_drv, pool.size() * sizeof(DrvRow)));
-    HIPCHK(hipMemcpyAsync(h->d_sets, sets.data(), sets.size() * sizeof(P18), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_sets, sets.data(), sets.size() * sizeof(PSet), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->d_wave_set, wave_set.data(), wave_set.size() * sizeof(int), hipMemcpyHostToDevice,
                           h->stream));
     HIPCHK(hipMemcpyAsync(h->d_slot_user, slot_user.data(), (size_t)S * sizeof(int), hipMemcpyHostToDevice,
@@ -298,7 +310,7 @@ std::vector<int> output_steps(int nsteps, double step, double dtout) {
 
 int launch(lfdm_solver* h, int mode, double t0, int nsteps, const std::vector<int>* outs, double cs = 0,
            double ct = 0, double cb = 0, const double* steps_ptr = nullptr) {
-    Launch18 L{};
+    Launch L{};
     L.sets = h->d_sets;
     L.wave_set = h->d_wave_set;
     L.slot_user = h->d_slot_user;
@@ -338,7 +350,9 @@ int launch(lfdm_solver* h, int mode, double t0, int nsteps, const std::vector<in
     int block = (int)((h->tpb + kWave - 1) / kWave) * kWave;
     block = std::min(256, std::max(64, block));
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    hipError_t e = launch_d18(L, block, h->stream);
+    hipError_t e = h->dof == 11 ? launch_d11(L, block, h->stream)
+                   : h->dof == 18 ? launch_d18(L, block, h->stream)
+                                  : launch_d24(L, block, h->stream);
     if (e != hipSuccess) return fail(LFDM_E_HIP, std::string("integration launch failed: ") + hipGetErrorString(e));
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     h->timed = true;
@@ -349,13 +363,13 @@ int launch(lfdm_solver* h, int mode, double t0, int nsteps, const std::vector<in
 
 void csv_header(lfdm_solver* h) {
     const unsigned nw = h->store_all ? h->total : h->num_outs;
-    const char* cols[13] = {"time", "x", "y", "vx", "vy", "roll", "yaw", "roll_rate", "yaw_rate",
-                            "wlf", "wrf", "wlr", "wrr"};
+    const int nc = out_channels(h->dof);
+    const char* const* cols = h->dof == 11 ? kCols11 : (h->dof == 18 ? kCols18 : kCols24);
     for (unsigned i = 0; i < nw; ++i) {
         std::ostringstream& o = h->csv[i];
-        for (const char* c : cols) o << c << ",";
+        for (int c = 0; c < nc; ++c) o << cols[c] << ",";
         o << std::endl;
-        for (int c = 0; c < 13; ++c) o << 0 << ",";
+        for (int c = 0; c < nc; ++c) o << 0 << ",";
         o << std::endl;
     }
 }
@@ -363,12 +377,13 @@ void csv_header(lfdm_solver* h) {
 void csv_rows(lfdm_solver* h, const std::vector<double>& host, unsigned rows) {
     const unsigned nw = h->store_all ? h->total : h->num_outs;
     const size_t N = h->total;
+    const int nc = out_channels(h->dof);
     for (unsigned i = 0; i < nw; ++i) {
         const unsigned v = h->store_all ? i : h->which_outs[i];
         std::ostringstream& o = h->csv[i];
         for (unsigned r = 0; r < rows; ++r) {
-            const size_t off = (size_t)r * N * 13;
-            for (int c = 0; c < 13; ++c) o << host[off + N * c + v] << ",";
+            const size_t off = (size_t)r * N * nc;
+            for (int c = 0; c < nc; ++c) o << host[off + N * c + v] << ",";
             o << std::endl;
         }
     }
@@ -402,16 +417,14 @@ int lfdm_record_len(int dof, int tire_type) {
 
 int lfdm_params_from_json(int dof, const char* veh, const char* tire, const char* sus, int tire_type, double* out,
                           int cap, int* len) {
-    (void)sus;
     if (!veh || !tire || !len) return fail(LFDM_E_ARG, "null argument");
     if (tire_type != LFDM_TIRE_TMEASY && tire_type != LFDM_TIRE_TMEASYNR) return fail(LFDM_E_ARG, "bad tire type");
+    if (dof != 11 && dof != 18 && dof != 24)
+        return fail(LFDM_E_UNSUPPORTED, "unsupported model: " + std::to_string(dof) + "-DOF");
+    if (dof == 24 && !sus) return fail(LFDM_E_ARG, "the 24-DOF model needs a suspension JSON file");
     std::vector<double> flat;
     std::string err;
-    bool ok;
-    if (dof == 18)
-        ok = params18_from_json(veh, tire, tire_type, flat, err);
-    else
-        return fail(LFDM_E_UNSUPPORTED, "unsupported model: " + std::to_string(dof) + "-DOF");
+    const bool ok = params_from_json(dof, veh, tire, sus ? sus : "", tire_type, flat, err);
     if (!ok) return fail(err.rfind("cannot open", 0) == 0 ? LFDM_E_IO : LFDM_E_PARSE, err);
     *len = (int)flat.size();
     if (!out || cap < (int)flat.size()) return fail(LFDM_E_CAPACITY, "output buffer too small");
@@ -433,7 +446,8 @@ int lfdm_driver_file_load(const char* path, double* rows, int cap_rows, int* nro
 int lfdm_create(int dof, unsigned total, lfdm_solver** out) {
     if (!out) return fail(LFDM_E_ARG, "null output handle");
     *out = nullptr;
-    if (dof != 18) return fail(LFDM_E_UNSUPPORTED, "unsupported model: " + std::to_string(dof) + "-DOF");
+    if (dof != 11 && dof != 18 && dof != 24)
+        return fail(LFDM_E_UNSUPPORTED, "unsupported model: " + std::to_string(dof) + "-DOF");
     if (total == 0) return fail(LFDM_E_ARG, "total vehicle count must be positive");
     lfdm_solver* h = new lfdm_solver();
     h->dof = dof;
@@ -479,7 +493,7 @@ int lfdm_construct_flat(lfdm_solver* h, const double* flat, int len, int tire_ty
     g.flat.assign(flat, flat + len);
     g.tire_nr = tire_type;
     std::string err;
-    if (!p18_from_flat(flat, len, tire_type, g.p18, err)) return fail(LFDM_E_ARG, err);
+    if (!pset_from_flat(h->dof, flat, len, tire_type, g.pset, err)) return fail(LFDM_E_ARG, err);
     if (rows && nrows > 0) g.rows.assign(reinterpret_cast<const DrvRow*>(rows), reinterpret_cast<const DrvRow*>(rows) + nrows);
     g.n = n;
     g.user_base = h->n_params;
@@ -505,17 +519,33 @@ int lfdm_construct(lfdm_solver* h, const char* veh, const char* tire, const char
     return lfdm_construct_flat(h, buf, len, tire_type, n, rows.empty() ? nullptr : &rows[0].t, (int)rows.size());
 }
 
-int lfdm_initialize(lfdm_solver* h, const double* record, unsigned n) {
-    if (int rc = check(h)) return rc;
-    if (!record) return fail(LFDM_E_ARG, "null record");
+namespace {
+
+// Appends n vehicles whose records are rec(i) (stride 0: one record for all, as Initialize).
+int append_states(lfdm_solver* h, const double* records, unsigned n, size_t stride) {
+    if (!records) return fail(LFDM_E_ARG, "null record");
     if (h->tire_type < 0) return fail(LFDM_E_STATE, "Initialize before Construct");
     if ((unsigned long long)n + h->n_states > h->total)
         return fail(LFDM_E_CAPACITY, "Initialize exceeds the total vehicle count of the solver");
-    for (int i = 0; i < h->rec; ++i)
-        if (!std::isfinite(record[i])) return fail(LFDM_E_ARG, "initial state holds a non-finite value");
+    for (unsigned i = 0; i < (stride ? n : 1u); ++i)
+        for (int f = 0; f < h->rec; ++f)
+            if (!std::isfinite(records[i * stride + f])) return fail(LFDM_E_ARG, "initial state holds a non-finite value");
     const unsigned first = h->n_states;
-    for (unsigned i = 0; i < n; ++i) h->init.insert(h->init.end(), record, record + h->rec);
+    if (h->dof == 24 && (unsigned long long)first + n > h->n_params)
+        return fail(LFDM_E_STATE, "24-DOF Initialize needs its vehicles constructed first");
+    for (unsigned i = 0; i < n; ++i) h->init.insert(h->init.end(), records + i * stride, records + i * stride + h->rec);
     h->n_states += n;
+    if (h->dof == 24) {
+        // initializeTireSus per vehicle (dof24_halfImplicit_gpu.cu:320-333), with the vehicle's own
+        // mass when per-vehicle parameters were set before Initialize
+        size_t gi = 0;
+        for (unsigned i = first; i < first + n; ++i) {
+            while (i >= h->groups[gi].user_base + h->groups[gi].n) ++gi;
+            const Group& G = h->groups[gi];
+            const double m = h->ovr_host.empty() ? G.pset.m : h->ovr_host[i];
+            init_tire_sus24(G.pset, m, h->tire_type == LFDM_TIRE_TMEASYNR, h->init.data() + (size_t)i * h->rec);
+        }
+    }
     if (h->built) {
         // the fleet is already on the device: write these records into their slots
         std::vector<int> slots(n);
@@ -531,6 +561,18 @@ int lfdm_initialize(lfdm_solver* h, const double* record, unsigned n) {
         HIPCHK(hipStreamSynchronize(h->stream));
     }
     return LFDM_OK;
+}
+
+}  // namespace
+
+int lfdm_initialize(lfdm_solver* h, const double* record, unsigned n) {
+    if (int rc = check(h)) return rc;
+    return append_states(h, record, n, 0);
+}
+
+int lfdm_initialize_batch(lfdm_solver* h, const double* records, unsigned n) {
+    if (int rc = check(h)) return rc;
+    return append_states(h, records, n, (size_t)h->rec);
 }
 
 int lfdm_set_end_time(lfdm_solver* h, double v) {
@@ -619,14 +661,15 @@ int lfdm_solve(lfdm_solver* h) {
                 return fail(LFDM_E_STATE, "No controls provided, please use construct to pass path to driver inputs");
     const double* steps = h->steps_ext ? h->steps_ext : h->d_steps;
     const size_t N = h->total;
+    const size_t nc = (size_t)out_channels(h->dof);
     unsigned dct = 0, hct = 0;
     std::vector<double> host;
     if (h->output) {
         dct = (unsigned)std::ceil(h->ksim / h->dtout);
         hct = (unsigned)std::ceil(h->host_dump / h->dtout);
-        if (int rc = ensure_buffer(h->d_out, h->out_cap, (size_t)dct * 13 * N)) return rc;
-        HIPCHK(hipMemsetAsync(h->d_out, 0, (size_t)dct * 13 * N * sizeof(double), h->stream));
-        host.assign((size_t)std::max(hct, dct) * 13 * N, 0.0);
+        if (int rc = ensure_buffer(h->d_out, h->out_cap, (size_t)dct * nc * N)) return rc;
+        HIPCHK(hipMemsetAsync(h->d_out, 0, (size_t)dct * nc * N * sizeof(double), h->stream));
+        host.assign((size_t)std::max(hct, dct) * nc * N, 0.0);
         for (auto& s : h->csv) { s.str(""); s.clear(); }
         csv_header(h);
     }
@@ -651,9 +694,9 @@ int lfdm_solve(lfdm_solver* h) {
         current = kend;
         since += h->ksim;
         if (h->output) {
-            const size_t filled = N * 13 * dct * launches;
-            if (host.size() < filled + (size_t)dct * 13 * N) host.resize(filled + (size_t)dct * 13 * N, 0.0);
-            HIPCHK(hipMemcpyAsync(host.data() + filled, h->d_out, (size_t)dct * 13 * N * sizeof(double),
+            const size_t filled = N * nc * dct * launches;
+            if (host.size() < filled + (size_t)dct * nc * N) host.resize(filled + (size_t)dct * nc * N, 0.0);
+            HIPCHK(hipMemcpyAsync(host.data() + filled, h->d_out, (size_t)dct * nc * N * sizeof(double),
                                   hipMemcpyDeviceToHost, h->stream));
             HIPCHK(hipStreamSynchronize(h->stream));
             ++launches;

@@ -57,6 +57,7 @@ SIGNATURES = {
     "lfdm_construct": [_p, _s, _s, _s, _u, _s, _i],
     "lfdm_construct_flat": [_p, _p, _i, _i, _u, _p, _i],
     "lfdm_initialize": [_p, _p, _u],
+    "lfdm_initialize_batch": [_p, _p, _u],
     "lfdm_set_end_time": [_p, _d],
     "lfdm_set_time_step": [_p, _d],
     "lfdm_set_kernel_sim_time": [_p, _d],

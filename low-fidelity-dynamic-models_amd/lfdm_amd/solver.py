@@ -1,7 +1,8 @@
 """Python mirror of the reference GPU solver classes over the C ABI.
 
 ``SolverHalfImplicitGPU(dof, total)`` follows d18SolverHalfImplicitGPU
-(wheeled_vehicle_models/18dof-gpu/dof18_halfImplicit_gpu.cuh:15-343): Construct, Initialize,
+(wheeled_vehicle_models/18dof-gpu/dof18_halfImplicit_gpu.cuh:15-343) and, for dof 11 and 24,
+d11SolverHalfImplicitGPU / d24SolverHalfImplicitGPU: Construct, Initialize,
 the Set*/Get* methods, SetOutput, Solve, SolveStep and GetSimState keep the reference's names,
 argument order and meaning.  States are exchanged as full records (numpy float64 arrays in the
 order documented in include/lfdm_amd.h); batched methods move whole fleets at once.
@@ -15,22 +16,46 @@ from ._lib import LFDM_TIRE_TMEASY, LFDM_TIRE_TMEASYNR, check, enc, lib
 TMeasy = LFDM_TIRE_TMEASY
 TMeasyNr = LFDM_TIRE_TMEASYNR
 
-# Field names of the canonical 18-DOF record (include/lfdm_amd.h).
+# Field names of the records (include/lfdm_amd.h).
 TIRE_FIELDS = ("xe", "ye", "xedot", "yedot", "omega", "dOmega", "xt", "rStat", "fx", "fy", "fz", "vsx", "vsy",
                "My", "engTor")
 TIRE_FIELDS_NR = TIRE_FIELDS[4:]
 VEH_FIELDS = ("x", "y", "u", "v", "psi", "wz", "phi", "wx", "udot", "vdot", "wxdot", "wzdot", "crankOmega",
               "dOmega_crank", "current_gr")
 TIRES = ("lf", "rf", "lr", "rr")
+VEH_FIELDS_11 = ("x", "y", "u", "v", "psi", "wz", "udot", "vdot", "wzdot", "crankOmega", "dOmega_crank", "current_gr")
+TIRES_11 = ("f", "r")
+TIRE_FIELDS_24 = ("xe", "ye", "xedot", "yedot", "omega", "dOmega", "xt", "dxt", "rStat", "fx", "fy", "fz", "fxg",
+                  "fyg", "fzg", "vgx", "vgy", "vgz", "vsx", "vsy", "vsz", "My", "engTor")
+TIRE_FIELDS_24_NR = TIRE_FIELDS_24[4:]
+SUS_FIELDS = ("xs", "xsi", "dxs", "ls", "lsi", "us", "vs", "ws", "dus", "dvs", "dws", "uu", "vu", "wu", "duu", "dvu",
+              "dwu", "fxs", "fys", "fzs", "fzd", "mx", "my", "mz")
+VEH_FIELDS_24 = ("x", "y", "z", "u", "v", "w", "psi", "wz", "phi", "wx", "theta", "wy", "udot", "vdot", "wdot",
+                 "wxdot", "wydot", "wzdot", "dtheta", "dphi", "dpsi", "crankOmega", "dOmega_crank", "current_gr")
 OVERRIDE_FIELDS = ("m", "jz", "jx", "dfy0Pn", "dfy0P2n", "torque_scale", "maxBrakeTorque")
+# Output channels of SetOutput / Solve (the reference's device response) and their CSV columns.
+OUTPUT_COLUMNS = {
+    11: ("time", "x", "y", "vx", "vy", "yaw", "yaw_rate", "wf", "wr"),
+    18: ("time", "x", "y", "vx", "vy", "roll", "yaw", "roll_rate", "yaw_rate", "wlf", "wrf", "wlr", "wrr"),
+    24: ("time", "x", "y", "vx", "vy", "roll", "yaw", "pitch", "roll_rate", "yaw_rate", "pitch_rate", "wlf", "wrf",
+         "wlr", "wrr"),
+}
 
 
 def record_fields(dof=18, tire_type=TMeasy):
-    """Names of the record entries, e.g. 'lf.omega' or 'x'."""
-    if dof != 18:
-        raise ValueError(f"unsupported model: {dof}-DOF")
-    tf = TIRE_FIELDS_NR if tire_type == TMeasyNr else TIRE_FIELDS
-    return [f"{t}.{f}" for t in TIRES for f in tf] + list(VEH_FIELDS)
+    """Names of the record entries, e.g. 'lf.omega', 'rr.fzd' (24-DOF strut) or 'x'."""
+    nr = tire_type == TMeasyNr
+    if dof == 18:
+        tf = TIRE_FIELDS_NR if nr else TIRE_FIELDS
+        return [f"{t}.{f}" for t in TIRES for f in tf] + list(VEH_FIELDS)
+    if dof == 11:
+        tf = TIRE_FIELDS_NR if nr else TIRE_FIELDS
+        return [f"{t}.{f}" for t in TIRES_11 for f in tf] + list(VEH_FIELDS_11)
+    if dof == 24:
+        tf = TIRE_FIELDS_24_NR if nr else TIRE_FIELDS_24
+        return ([f"{t}.{f}" for t in TIRES for f in tf] + [f"sus_{t}.{f}" for t in TIRES for f in SUS_FIELDS] +
+                list(VEH_FIELDS_24))
+    raise ValueError(f"unsupported model: {dof}-DOF")
 
 
 def record_index(name, dof=18, tire_type=TMeasy):
@@ -92,10 +117,15 @@ class SolverHalfImplicitGPU:
         self.close()
 
     # -- construction ------------------------------------------------------------------
-    def Construct(self, veh_params_file, tire_params_file, num_vehicles, driver_file=None, tire_type=TMeasy,
-                  sus_params_file=None):
-        check(lib.lfdm_construct(self._h, enc(veh_params_file), enc(tire_params_file), enc(sus_params_file),
-                                 num_vehicles, enc(driver_file), tire_type))
+    def Construct(self, veh_params_file, tire_params_file, *args, **kw):
+        """Construct(veh, tire, n[, driver_file][, tire_type]) for 11/18-DOF and
+        Construct(veh, tire, sus, n[, driver_file][, tire_type]) for 24-DOF, as in the reference."""
+        names = (["sus_params_file"] if self.dof == 24 else []) + ["num_vehicles", "driver_file", "tire_type"]
+        a = dict(zip(names, args))
+        a.update(kw)
+        tire_type = a.get("tire_type", TMeasy)
+        check(lib.lfdm_construct(self._h, enc(veh_params_file), enc(tire_params_file), enc(a.get("sus_params_file")),
+                                 a["num_vehicles"], enc(a.get("driver_file")), tire_type))
         self.tire_type = tire_type
 
     def construct_flat(self, flat, num_vehicles, driver_rows=None, tire_type=TMeasy):
@@ -109,6 +139,11 @@ class SolverHalfImplicitGPU:
     def Initialize(self, record, num_vehicles):
         rec = _f64(record)
         check(lib.lfdm_initialize(self._h, rec.ctypes.data, num_vehicles))
+
+    def initialize_batch(self, records):
+        """Appends len(records) vehicles, each with its own record (rows of an (n, rec) array)."""
+        rec = _f64(records)
+        check(lib.lfdm_initialize_batch(self._h, rec.ctypes.data, rec.shape[0]))
 
     def zero_record(self):
         return np.zeros(record_len(self.dof, self.tire_type or TMeasy))

@@ -110,6 +110,30 @@ def knot_fleet(n, flat, seed=2, horizon=10.0, max_steer=None, jitter=0.1, dof=18
     return dict(ns=ns, ds=ds, np=np_, dp=dp, knots=np.ascontiguousarray(knots), ovr=np.ascontiguousarray(ovr))
 
 
+def initial_speed_records(n, flat, dof=24, seed=7, umax=20.0, tire_nr=False):
+    """Initial records (n, rec) of a fleet that starts moving: forward speed u0 ~ U(0, umax) per
+    vehicle, every wheel rolling at u0 / r0, first gear, crank at the wheel speed over the first
+    gear ratio (the transmission's output-to-input relation, omega_out = omega_wheel / ratio).
+    Everything else is zero; for the 24-DOF model Initialize then adds the static tire and strut
+    state (initializeTireSus)."""
+    from .solver import record_fields
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    u0 = rng.uniform(0.0, umax, n)
+    F = FLAT[dof]
+    gr0 = flat[F["gears"] + 1]
+    r0 = flat[tire_offsets(flat, tire_nr, dof)[0] - 25 + 3]  # tire block: jw rr mu r0 ...
+    fields = record_fields(dof, 1 if tire_nr else 0)
+    rec = np.zeros((n, len(fields)))
+    rec[:, fields.index("u")] = u0
+    om = u0 / r0
+    for i, f in enumerate(fields):
+        if f.endswith(".omega"):
+            rec[:, i] = om
+    rec[:, fields.index("crankOmega")] = om / gr0
+    return rec
+
+
 def knot_controls(t, ns, ds, np_, dp, knots):
     """Per-step controls (len(t), 3, n) of a knot program, evaluated with the solver's formula:
     steering lerped on the ds grid (segment index floor(t / ds) clamped to [0, ns - 2], weight

@@ -230,6 +230,37 @@ int orc18_count_ops(int nr, const double* flat, const double* init, double t0, d
     return 0;
 }
 
+// Same tallies for any model (dof 11, 18 or 24) from an initial record (NULL: zero state).
+int orc_count_ops(int dof, int nr, const double* flat, const double* init, double t0, double dt, int nsteps, int mode,
+                  const double* ctrl, const int* imeta, const double* dmeta, long long* counts) {
+    auto run = [&](auto model) {
+        using M = decltype(model);
+        const int rl = M::rec(nr != 0);
+        Controls cs{mode, ctrl, imeta, dmeta, 1};
+        typename M::template V<CD> s;
+        s.nr = nr != 0;
+        M::params(s, flat);
+        std::vector<double> z(rl, 0.0);
+        M::set(s, init ? init : z.data());
+        g_ops = OpCount{};
+        double t = t0;
+        for (int k = 0; k < nsteps; ++k) {
+            double st, th, br;
+            cs.at(0, k, t, st, th, br);
+            M::step(s, CD(st), CD(th), CD(br), dt);
+            t = t + dt;
+        }
+    };
+    if (dof == 11) run(M11{});
+    else if (dof == 18) run(M18{});
+    else if (dof == 24) run(M24{});
+    else return -1;
+    const long long c[] = {g_ops.add, g_ops.mul,  g_ops.div,   g_ops.sqrt,  g_ops.sin,
+                           g_ops.cos, g_ops.tan,  g_ops.atan2, g_ops.hypot, g_ops.tanh};
+    std::memcpy(counts, c, sizeof(c));
+    return 0;
+}
+
 // ---- building blocks, exposed for closed-form and invariant checks ----
 
 void orc_tmxy(double s, double df0, double sm, double fm, double ss, double fs, double* f, double* fos) {

@@ -41,6 +41,8 @@ def lib():
         L.orc24_init.restype = i
         L.orc18_count_ops.argtypes = [i, p, p, d, d, i, i, p, p, p, p]
         L.orc18_count_ops.restype = i
+        L.orc_count_ops.argtypes = [i, i, p, p, d, d, i, i, p, p, p, p]
+        L.orc_count_ops.restype = i
         L.orc_tmxy.argtypes = [d, d, d, d, d, d, p, p]
         L.orc_tmxy.restype = None
         for fn in ("orc_interp_l", "orc_interp_q"):
@@ -125,6 +127,18 @@ def count_ops18(flat, controls, nsteps, tire_nr=False, dt=1e-3):
     c = np.zeros(len(OP_NAMES), dtype=np.int64)
     L.orc18_count_ops(int(tire_nr), _ptr(flat), None, 0.0, dt, nsteps, mode, _ptr(ctrl), _ptr(im), _ptr(dm),
                       _ptr(c))
+    return dict(zip(OP_NAMES, c.tolist()))
+
+
+def count_ops(dof, flat, controls, nsteps, tire_nr=False, init=None, dt=1e-3):
+    """Operation tallies over nsteps of one vehicle of the 11-, 18- or 24-DOF model."""
+    L = lib()
+    flat = np.ascontiguousarray(flat, dtype=np.float64)
+    init = None if init is None else np.ascontiguousarray(init, dtype=np.float64)
+    mode, ctrl, im, dm = _ctrl(controls)
+    c = np.zeros(len(OP_NAMES), dtype=np.int64)
+    assert L.orc_count_ops(dof, int(tire_nr), _ptr(flat), _ptr(init), 0.0, dt, nsteps, mode, _ptr(ctrl), _ptr(im),
+                           _ptr(dm), _ptr(c)) == 0
     return dict(zip(OP_NAMES, c.tolist()))
 
 

@@ -11,6 +11,7 @@ import lfdm_amd
 from lfdm_amd import LfdmError, SolverHalfImplicitGPU, TMeasy, TMeasyNr
 from lfdm_amd._lib import lib
 from cases18 import CASES, paths
+from cases1124 import CASES as CASES1124, paths as paths1124
 from conftest import DATA, ROOT
 
 HEADER = os.path.join(ROOT, "include", "lfdm_amd.h")
@@ -39,11 +40,25 @@ def test_params_match_reference_parser(name, golden18):
     np.testing.assert_array_equal(flat, golden18[f"{name}/params"])
 
 
+@pytest.mark.parametrize("dof,name", [(d, n) for d in (11, 24) for n in sorted(CASES1124[d])])
+def test_params_match_reference_parser_11_24(dof, name, request):
+    """The 11-DOF loaders double the axle tire's stiffness, damping, inertia and force parameters
+    (dof11.cpp:1036-1380); the 24-DOF ones append the suspension file (dof24.cpp:2192-2216)."""
+    g = request.getfixturevalue(f"golden{dof}")
+    veh, tire, nr, _, _, sus = paths1124(dof, name)
+    flat = lfdm_amd.params_from_json(dof, veh, tire, sus, tire_type=TMeasyNr if nr else TMeasy)
+    np.testing.assert_array_equal(flat, g[f"{name}/params"])
+
+
 def test_record_layout():
     assert lfdm_amd.record_len(18, TMeasy) == 75
     assert lfdm_amd.record_len(18, TMeasyNr) == 59
     f = lfdm_amd.record_fields(18, TMeasy)
     assert f[4] == "lf.omega" and f[60] == "x" and f[-1] == "current_gr"
+    for dof, nr, n in ((11, TMeasy, 42), (11, TMeasyNr, 34), (24, TMeasy, 212), (24, TMeasyNr, 196)):
+        assert lfdm_amd.record_len(dof, nr) == n == len(lfdm_amd.record_fields(dof, nr))
+    f24 = lfdm_amd.record_fields(24, TMeasy)
+    assert f24[4] == "lf.omega" and f24[92] == "sus_lf.xs" and f24[-24] == "x" and f24[-1] == "current_gr"
     with pytest.raises(LfdmError):
         lfdm_amd.record_len(7, TMeasy)
 
@@ -66,6 +81,14 @@ def test_parse_errors_are_reported():
     with pytest.raises(LfdmError) as e:
         lfdm_amd.load_driver_file(f"{d}/HMMWV/vehicle.json")  # JSON is no driver table
     assert e.value.name == "LFDM_E_IO"
+    # the bundled 24-DOF Sedan vehicle_RWD.json is an 18-DOF file (no "jy"): the reference aborts
+    d24 = os.path.join(DATA, "24dof", "Sedan")
+    with pytest.raises(LfdmError) as e:
+        lfdm_amd.params_from_json(24, f"{d24}/vehicle_RWD.json", f"{d24}/tmeasy.json", f"{d24}/suspension.json")
+    assert e.value.name == "LFDM_E_PARSE" and "jy" in str(e.value)
+    with pytest.raises(LfdmError) as e:  # the 24-DOF model needs its suspension file
+        lfdm_amd.params_from_json(24, f"{d24}/vehicle.json", f"{d24}/tmeasy.json")
+    assert e.value.name == "LFDM_E_ARG"
 
 
 def test_handle_lifecycle_errors(tmp_path):
@@ -125,3 +148,24 @@ def test_errors_without_gpu_are_loud():
     with pytest.raises(LfdmError) as e:
         s.Solve()
     assert e.value.name == "LFDM_E_HIP"
+
+
+@pytest.mark.parametrize("name", sorted(CASES1124[24]))
+def test_initialize24_runs_tire_suspension_init(name, golden24):
+    """Initialize on a 24-DOF handle applies initializeTireSus (dof24_halfImplicit_gpu.cu:320-333)
+    on the host: zero records plus the case's initial speed give the reference's initial records bit
+    for bit, with the per-vehicle masses of the knot cases (set before Initialize)."""
+    from cases1124 import inputs
+
+    flat, _, n, ovr, init = inputs(24, name, golden24)
+    nr = bool(golden24[f"{name}/spec"][2])
+    tt = TMeasyNr if nr else TMeasy
+    s = SolverHalfImplicitGPU(24, n)
+    s.construct_flat(flat, n, None, tt)
+    if ovr is not None:
+        s.set_overrides(ovr)
+    rec = s.zero_record()
+    rec[-24 + 3] = golden24[f"{name}/spec"][3]
+    s.Initialize(rec, n)
+    np.testing.assert_array_equal(s.get_states(), init)
+    s.close()
