@@ -78,6 +78,7 @@ struct M18 {
     using X = X18;
     using K = K18;
     static constexpr int kCh = 13;
+    static constexpr int kWaves = 2;
     template <bool NR>
     struct R {
         static constexpr int kTire = NR ? 11 : 15, kVeh = 4 * kTire, om = NR ? 0 : 4;
@@ -149,6 +150,7 @@ struct M11 {
     using X = X11;
     using K = K11;
     static constexpr int kCh = 9;
+    static constexpr int kWaves = 2;
     template <bool NR>
     struct R {
         static constexpr int kTire = NR ? 11 : 15, kVeh = 2 * kTire, om = NR ? 0 : 4;
@@ -220,6 +222,7 @@ struct M24 {
     using X = X24;
     using K = K24;
     static constexpr int kCh = 15;
+    static constexpr int kWaves = 1;
     template <bool NR>
     struct R {
         static constexpr int kTire = NR ? 19 : 23, kSus = 4 * kTire, kVeh = kSus + 4 * 24, om = NR ? 0 : 4;
@@ -316,21 +319,21 @@ struct M24 {
     }
 };
 
-// Occupancy target of the integration kernels (waves per SIMD); 0 leaves it to the compiler.
-// The 18-DOF step needs more registers than two waves per SIMD leave (the compiler spills some to
-// scratch), yet two resident waves hide more FP64 latency than the spills cost: 5.0e9 versus
-// 3.8e9 vehicle-steps/s at one wave per SIMD on the 1M-vehicle fleet.
-#ifndef LFDM_WAVES_PER_EU
-#define LFDM_WAVES_PER_EU 2
-#endif
-#if LFDM_WAVES_PER_EU > 0
-#define LFDM_OCCUPANCY __attribute__((amdgpu_waves_per_eu(LFDM_WAVES_PER_EU, LFDM_WAVES_PER_EU)))
+// Occupancy target of the integration kernels (waves per SIMD), per model.  The 11- and 18-DOF
+// steps need more registers than two waves per SIMD leave (the compiler spills some to scratch),
+// yet two resident waves hide more FP64 latency than the spills cost (18-DOF, 1M vehicles: 5.0e9
+// versus 3.7e9 vehicle-steps/s at one wave; 11-DOF 9.4e9 versus 5.8e9).  The 24-DOF step would
+// spill over a kilobyte per lane at two waves and runs 2.6x faster at one (3.3e9 versus 1.3e9).
+// LFDM_WAVES_PER_EU overrides every model (0 leaves it to the compiler).
+#ifdef LFDM_WAVES_PER_EU
+#define LFDM_WAVES(M) LFDM_WAVES_PER_EU
 #else
-#define LFDM_OCCUPANCY
+#define LFDM_WAVES(M) (M::kWaves)
 #endif
 
 template <class M, bool NR, int MODE>
-__global__ __launch_bounds__(256) LFDM_OCCUPANCY void integrate(Launch L) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LFDM_WAVES(M), LFDM_WAVES(M)))) void integrate(
+    Launch L) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int wave = uniform(slot >> 6);
     if (wave * kWave >= L.S) return;
