@@ -82,6 +82,19 @@ def cpu_baseline(dof, flat, n_sample, nsteps, threads):
     return n_sample * nsteps / dt, n1 * nsteps / d1, dt
 
 
+def measured_traffic_per_step():
+    """HBM bytes per vehicle-step of the 18-DOF integration kernel from the calibrated FETCH_SIZE +
+    WRITE_SIZE counter runs summarized in profiles/ (None if the summary is absent)."""
+    path = os.path.join(ROOT, "profiles", "integrate_18dof_knot_fleet_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_vehicle_step:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -159,10 +172,15 @@ def main():
     kernel_ms = float(np.mean(launch_ms))
     fl_total, fl_basic = flops_per_step(dof, flat)
     achieved_tf = fl_total * n * H / (kernel_ms * 1e-3) / 1e12
+    tps = measured_traffic_per_step() if dof == 18 else None
     roofline = {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
+                "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None if tps is None else tps * n * H,
                 "kernel": f"integrate<M{dof}, TMeasy, knots>", "kernel_ms": kernel_ms,
-                "flop_per_vehicle_step": fl_total, "basic_flop_per_vehicle_step": fl_basic}
+                "flop_per_vehicle_step": fl_total, "basic_flop_per_vehicle_step": fl_basic,
+                "traffic_source": "profiles/integrate_18dof_knot_fleet_counters.md (bytes per vehicle-step x "
+                                  "vehicle-steps of one launch)" if tps is not None else None,
+                "hbm_gbs": None if tps is None else tps * n * H / (kernel_ms * 1e-3) / 1e9,
+                "hbm_frac": None if tps is None else tps * n * H / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -97,6 +97,7 @@ int lfdm_set_host_dump_time(lfdm_solver* h, double time);
 int lfdm_set_threads_per_block(lfdm_solver* h, unsigned threads);
 int lfdm_get_step(lfdm_solver* h, double* step);
 int lfdm_get_end_time(lfdm_solver* h, double* tend);
+int lfdm_get_kernel_sim_time(lfdm_solver* h, double* time);
 int lfdm_get_tire_type(lfdm_solver* h, int* tire_type);
 
 /* SetOutput(file, freq, store_all, no_outs), .cuh:138-141: CSV files <file>_<vehicle>.csv */
@@ -131,6 +132,15 @@ int lfdm_set_controls_knots(lfdm_solver* h, int ns, double ds, int np, double dp
 int lfdm_set_controls_steps(lfdm_solver* h, int nsteps, const double* ctrl);
 int lfdm_set_controls_steps_device(lfdm_solver* h, int nsteps, const double* device_ctrl);
 int lfdm_set_controls_table(lfdm_solver* h);
+/* Per-step controls shared by every vehicle, device_ctrl[nsteps][3] (steering, throttle, braking):
+ * what the reference's templated SolveStep(t, functor) evaluates inside its kernel
+ * (dof18_halfImplicit_gpu.cuh:233-290, 454-551); lfdm_amd_functor.hpp fills it from a functor. */
+int lfdm_set_controls_shared_device(lfdm_solver* h, int nsteps, const double* device_ctrl);
+/* Number of steps one kernel-sim-time chunk starting at t takes (the reference kernel's loop
+ * t < t0 + kernel_sim_time - step/10, dof18_halfImplicit_gpu.cu:892-948). */
+int lfdm_chunk_steps(lfdm_solver* h, double t, int* nsteps);
+/* Number of vehicles initialized so far (Initialize / initialize_batch / set_states). */
+int lfdm_initialized_count(lfdm_solver* h, unsigned* count);
 
 /* Advances every vehicle nsteps steps from its current time in one launch (no output), on the
  * handle's stream, without synchronizing.  The time base and the position in per-step controls

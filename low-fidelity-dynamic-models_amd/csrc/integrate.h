@@ -40,11 +40,18 @@ __device__ __forceinline__ Ctl controls_at(const Launch& L, const PSet& P, CtrlS
     } else if (MODE == kCtrlTable) {
         driver_input(t, L.drv + P.drv_off, P.drv_rows, c.s, c.th, c.b);
     } else if (MODE == kCtrlSteps) {
-        const size_t N = (size_t)L.n_user;
-        const double* p = L.steps + (size_t)k * 3 * N + uv;
-        c.s = p[0];
-        c.th = p[N];
-        c.b = p[2 * N];
+        if (L.steps_shared) {  // wave-uniform: one row per step for the whole fleet
+            const double* p = L.steps + (size_t)k * 3;
+            c.s = p[0];
+            c.th = p[1];
+            c.b = p[2];
+        } else {
+            const size_t N = (size_t)L.n_user;
+            const double* p = L.steps + (size_t)k * 3 * N + uv;
+            c.s = p[0];
+            c.th = p[N];
+            c.b = p[2 * N];
+        }
     } else {  // kCtrlKnots
         const size_t N = (size_t)L.n_user;
         int ks = (int)floor(t / L.ds);

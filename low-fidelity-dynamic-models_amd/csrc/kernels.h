@@ -28,6 +28,7 @@ struct Launch {
     double ds, dp;         // knot spacings (kCtrlKnots)
     int ns, np;
     int nsteps, n_out, S, n_user, mode, nr;
+    int steps_shared;      // kCtrlSteps: 1 = steps[nsteps][3] shared by every vehicle
 };
 
 // Integration launches of the 11-, 18- and 24-DOF models (one kernel per tire model and control

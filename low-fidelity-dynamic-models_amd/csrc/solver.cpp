@@ -101,6 +101,7 @@ struct lfdm_solver {
     int steps_pos = 0;  // next per-step control row consumed by lfdm_advance
     std::vector<double> steps_host;  // [steps_n][3][total]
     const double* steps_ext = nullptr;
+    bool steps_shared = false;  // steps_ext is [steps_n][3] for the whole fleet
     bool steps_dirty = false;
     std::vector<double> ovr_host;  // [7][total]
     bool ovr_dirty = false;
@@ -333,6 +334,7 @@ int launch(lfdm_solver* h, int mode, double t0, int nsteps, const std::vector<in
     L.n_user = (int)h->total;
     L.mode = mode;
     L.nr = h->tire_type == LFDM_TIRE_TMEASYNR;
+    L.steps_shared = (mode == kCtrlSteps && h->steps_shared) ? 1 : 0;
     if (outs && !outs->empty()) {
         h->outs_host = *outs;
         outs = &h->outs_host;
@@ -617,6 +619,12 @@ int lfdm_get_end_time(lfdm_solver* h, double* v) {
     *v = h->tend;
     return LFDM_OK;
 }
+int lfdm_get_kernel_sim_time(lfdm_solver* h, double* v) {
+    if (int rc = check(h)) return rc;
+    if (!v) return fail(LFDM_E_ARG, "null output");
+    *v = h->ksim;
+    return LFDM_OK;
+}
 int lfdm_get_tire_type(lfdm_solver* h, int* v) {
     if (int rc = check(h)) return rc;
     if (!v) return fail(LFDM_E_ARG, "null output");
@@ -687,7 +695,8 @@ int lfdm_solve(lfdm_solver* h) {
         std::vector<int> outs;
         if (h->output) outs = output_steps(n, h->step, h->dtout);
         if (outs.size() > dct) outs.resize(dct);
-        const double* sp = mode == kCtrlSteps ? steps + (size_t)step_index * 3 * N : nullptr;
+        const size_t row = h->steps_shared ? 3 : 3 * N;
+        const double* sp = mode == kCtrlSteps ? steps + (size_t)step_index * row : nullptr;
         if (int rc = launch(h, mode, t_cont, n, &outs, 0, 0, 0, sp)) return rc;
         for (int k = 0; k < n; ++k) t_cont += h->step;
         step_index += n;
@@ -813,6 +822,7 @@ int lfdm_set_controls_steps(lfdm_solver* h, int nsteps, const double* ctrl) {
     h->steps_pos = 0;
     h->steps_host.assign(ctrl, ctrl + (size_t)nsteps * 3 * h->total);
     h->steps_ext = nullptr;
+    h->steps_shared = false;
     h->steps_dirty = true;
     h->ctrl_mode = kCtrlSteps;
     return LFDM_OK;
@@ -824,8 +834,35 @@ int lfdm_set_controls_steps_device(lfdm_solver* h, int nsteps, const double* dct
     h->steps_n = nsteps;
     h->steps_pos = 0;
     h->steps_ext = dctrl;
+    h->steps_shared = false;
     h->steps_dirty = false;
     h->ctrl_mode = kCtrlSteps;
+    return LFDM_OK;
+}
+
+int lfdm_set_controls_shared_device(lfdm_solver* h, int nsteps, const double* dctrl) {
+    if (int rc = check(h)) return rc;
+    if (!dctrl || nsteps <= 0) return fail(LFDM_E_ARG, "bad per-step controls");
+    h->steps_n = nsteps;
+    h->steps_pos = 0;
+    h->steps_ext = dctrl;
+    h->steps_shared = true;
+    h->steps_dirty = false;
+    h->ctrl_mode = kCtrlSteps;
+    return LFDM_OK;
+}
+
+int lfdm_chunk_steps(lfdm_solver* h, double t, int* nsteps) {
+    if (int rc = check(h)) return rc;
+    if (!nsteps) return fail(LFDM_E_ARG, "null output");
+    *nsteps = chunk_steps(t, h->ksim, h->step);
+    return LFDM_OK;
+}
+
+int lfdm_initialized_count(lfdm_solver* h, unsigned* count) {
+    if (int rc = check(h)) return rc;
+    if (!count) return fail(LFDM_E_ARG, "null output");
+    *count = h->n_states;
     return LFDM_OK;
 }
 
@@ -856,7 +893,8 @@ int lfdm_advance(lfdm_solver* h, int nsteps) {
                                       " more steps, " + std::to_string(nsteps) + " requested");
     const double* steps = nullptr;
     if (mode == kCtrlSteps)
-        steps = (h->steps_ext ? h->steps_ext : h->d_steps) + (size_t)h->steps_pos * 3 * h->total;
+        steps = (h->steps_ext ? h->steps_ext : h->d_steps) +
+                (size_t)h->steps_pos * (h->steps_shared ? 3 : 3 * (size_t)h->total);
     if (int rc = launch(h, mode, h->t_now, nsteps, nullptr, 0, 0, 0, steps)) return rc;
     if (mode == kCtrlSteps) h->steps_pos += nsteps;
     for (int k = 0; k < nsteps; ++k) h->t_now += h->step;

@@ -65,6 +65,7 @@ SIGNATURES = {
     "lfdm_set_threads_per_block": [_p, _u],
     "lfdm_get_step": [_p, _pd],
     "lfdm_get_end_time": [_p, _pd],
+    "lfdm_get_kernel_sim_time": [_p, _pd],
     "lfdm_get_tire_type": [_p, ctypes.POINTER(_i)],
     "lfdm_set_output": [_p, _s, _d, _i, _u],
     "lfdm_solve": [_p],
@@ -77,6 +78,9 @@ SIGNATURES = {
     "lfdm_set_controls_steps": [_p, _i, _p],
     "lfdm_set_controls_steps_device": [_p, _i, _p],
     "lfdm_set_controls_table": [_p],
+    "lfdm_set_controls_shared_device": [_p, _i, _p],
+    "lfdm_chunk_steps": [_p, _d, ctypes.POINTER(_i)],
+    "lfdm_initialized_count": [_p, ctypes.POINTER(_u)],
     "lfdm_advance": [_p, _i],
     "lfdm_set_time": [_p, _d],
     "lfdm_set_stream": [_p, _p],
