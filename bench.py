@@ -82,6 +82,62 @@ def cpu_baseline(dof, flat, n_sample, nsteps, threads):
     return n_sample * nsteps / dt, n1 * nsteps / d1, dt
 
 
+def adjoint_cost(flat, n=65536, T=5000, K=100, mpc_iterations=20):
+    """Differentiation form on the adjoint workload: 18-DOF HMMWV, n vehicles, T steps, tracking loss
+    to a seeded arc at 15 m/s (lfdm_amd.mpc), checkpoints every K steps.  Device times of the
+    adjoint's forward launch, of its backward launch and of the plain integration launch over the same
+    controls, from HIP events on the solver's stream; then `mpc_iterations` Adam iterations of MPC."""
+    import torch
+    from lfdm_amd.mpc import arc_reference, initial_controls, moving_start, run_mpc
+    from lfdm_amd.torch_ops import default_params, make_solver
+
+    dev = "cuda"
+    s = make_solver(flat, n)
+    s.set_stream(torch.cuda.current_stream().cuda_stream)
+    ref, kappa = arc_reference(n, T, device=dev)
+    init = moving_start(flat, n)
+    params = default_params(flat, n)
+    s.set_overrides(params.numpy())
+    s.set_states(init.numpy())
+    ctrl = initial_controls(flat, n, T, kappa, device=dev)
+    xy = torch.empty((T, 2, n), dtype=torch.float64, device=dev)
+    gc = torch.empty_like(ctrl)
+    gs = torch.empty((init.shape[1], n), dtype=torch.float64, device=dev)
+    gp = torch.empty((7, n), dtype=torch.float64, device=dev)
+    fwd, bwd = [], []
+    for it in range(3):  # the first round warms up
+        s.adjoint_forward(T, K, ctrl.data_ptr(), xy.data_ptr())
+        f_ms = s.last_launch_ms()
+        gxy = (2e-3 * (xy - ref)).contiguous()  # d/dxy of sum_k dt |xy_k - ref_k|^2
+        s.adjoint_backward(gxy.data_ptr(), gc.data_ptr(), gs.data_ptr(), gp.data_ptr())
+        b_ms = s.last_launch_ms()
+        if it:
+            fwd.append(f_ms)
+            bwd.append(b_ms)
+    s.set_controls_steps_device(T, ctrl.data_ptr())
+    plain = []
+    for it in range(2):
+        s.set_states(init.numpy())
+        s.set_time(0.0)
+        s.advance(T)
+        plain.append(s.last_launch_ms())
+    torch.cuda.synchronize()
+    f_ms, b_ms, p_ms = float(np.mean(fwd)), float(np.mean(bwd)), float(np.min(plain))
+    n_inputs = 3 * T + 6 + 23  # controls, parameters (m, Izz, Ixx, cornering stiffness, torque scale, brake), state
+    t0 = time.perf_counter()
+    _, hist = run_mpc(s, flat, init, params, ref, ctrl, iterations=mpc_iterations)
+    torch.cuda.synchronize()
+    mpc_s = time.perf_counter() - t0
+    s.close()
+    return {"vehicles": n, "steps": T, "checkpoint_every": K, "forward_ms": f_ms, "backward_ms": b_ms,
+            "ratio": (f_ms + b_ms) / f_ms, "plain_forward_ms": p_ms, "ratio_vs_plain_forward": (f_ms + b_ms) / p_ms,
+            "central_fd_cost_ratio": 2.0 * n_inputs * p_ms / (f_ms + b_ms),
+            "mpc": {"iterations": mpc_iterations, "seconds": mpc_s, "mean_loss_first": hist[0],
+                    "mean_loss_last": hist[-1]},
+            "note": "ratio = (adjoint forward + backward) / adjoint forward device time; central_fd_cost_ratio = "
+                    "cost of central differences (2 plain forwards per input) / adjoint cost"}
+
+
 def measured_traffic_per_step():
     """HBM bytes per vehicle-step of the 18-DOF integration kernel from the calibrated FETCH_SIZE +
     WRITE_SIZE counter runs summarized in profiles/ (None if the summary is absent)."""
@@ -118,6 +174,8 @@ def main():
     ap.add_argument("--tpb", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=32768)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation-form measurement")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the differentiation-form measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (reads 1 GiB, writes 1 GiB), the "
                          "reference kernel that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes")
@@ -140,6 +198,11 @@ def main():
     import lfdm_amd
     from lfdm_amd import SolverHalfImplicitGPU, TMeasy
     from lfdm_amd import workloads as W
+
+    if args.adjoint_only:
+        d = os.path.join(W.DATA, "18dof", "HMMWV")
+        print(json.dumps(adjoint_cost(lfdm_amd.params_from_json(18, f"{d}/vehicle.json", f"{d}/tmeasy.json"))))
+        return
 
     n, H, dof = args.vehicles, args.horizon_steps, args.dof
     d = os.path.join(W.DATA, f"{dof}dof", "HMMWV")
@@ -191,6 +254,9 @@ def main():
                "single_core": rate1, "cpu": cpu_model()}
     metric = ("18-DOF FP64 vehicle-steps/s at 1M vehicles; adjoint/forward cost ratio" if dof == 18 else
               f"{dof}-DOF FP64 vehicle-steps/s")
+    adj = None
+    if dof == 18 and not args.no_adjoint:
+        adj = adjoint_cost(flat)
     line = {"metric": metric, "value": value,
             "unit": "vehicle-steps/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -200,7 +266,8 @@ def main():
                                    "+-10% mass/Izz/cornering stiffness, knot controls" +
                                    (", start at U(0,20) m/s" if dof == 24 else ""),
                        "vehicles": n, "horizon_steps": H, "dt": 1e-3, "threads_per_block": args.tpb},
-            "roofline": roofline, "cpu_baseline": cpu}
+            "roofline": roofline, "cpu_baseline": cpu, "adjoint": adj,
+            "adjoint_forward_cost_ratio": None if adj is None else adj["ratio"]}
     print(json.dumps(line))
     s.close()
 

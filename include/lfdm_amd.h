@@ -150,6 +150,22 @@ int lfdm_advance(lfdm_solver* h, int nsteps);
  * starts at t = 0 from row 0). */
 int lfdm_set_time(lfdm_solver* h, double t);
 
+/* ---- differentiation form (no counterpart in the reference) ------------------------------
+ * Reverse-mode gradients of the 18-DOF half-implicit map (TMeasy tires, linear steering, no
+ * throttleMod; LFDM_E_UNSUPPORTED otherwise), with discrete events frozen at the forward pass.
+ * lfdm_adjoint_forward integrates nsteps steps from the handle's current records (which it does not
+ * modify) with per-step controls device_ctrl[nsteps][3][total], writes the position after every step
+ * to device_xy[nsteps][2][total] (x, y) and keeps a checkpoint of every vehicle's carried state every
+ * checkpoint_every steps.  lfdm_adjoint_backward takes dLoss/d(xy) in device_xy_bar (layout of
+ * device_xy) and writes dLoss/d(controls) (layout of device_ctrl), dLoss/d(initial record)
+ * [record_len][total] and dLoss/d(per-vehicle parameters) [7][total] in the order of
+ * lfdm_set_overrides.  device_ctrl stays borrowed until the backward call.  Both are asynchronous on
+ * the handle's stream; lfdm_last_launch_ms times the last of them. */
+int lfdm_adjoint_forward(lfdm_solver* h, int nsteps, int checkpoint_every, const double* device_ctrl,
+                         double* device_xy);
+int lfdm_adjoint_backward(lfdm_solver* h, const double* device_xy_bar, double* device_ctrl_bar,
+                          double* device_state_bar, double* device_param_bar);
+
 /* Stream used for every launch and copy (hipStream_t; NULL = the handle's own stream). */
 int lfdm_set_stream(lfdm_solver* h, void* stream);
 int lfdm_synchronize(lfdm_solver* h);

@@ -30,7 +30,7 @@ struct K11 {
     double Z1, Z3, Z4n, two_ab, mt;
 };
 
-__device__ __forceinline__ void k11_init(const PSet& P, double m, double jz, double dfy0Pn, double dfy0P2n,
+LFDM_HD void k11_init(const PSet& P, double m, double jz, double dfy0Pn, double dfy0P2n,
                                          double tqs, double maxBrake, K11& k) {
     const double a = P.a, b = P.b, muf = P.muf, mur = P.mur;
     const double huf = P.tire.r0, hur = P.tire.r0;
@@ -48,7 +48,7 @@ __device__ __forceinline__ void k11_init(const PSet& P, double m, double jz, dou
 
 // One half-implicit step (rhsFun order, dof11_halfImplicit.cpp:560-605; update :172-235).
 template <bool NR, bool FULL>
-__device__ __forceinline__ void step11(const PSet& P, const Derived& u, const K11& k, S11& s, double steering,
+LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, double steering,
                                        double throttle, double braking, double dt, X11* x) {
     const TireP& t = P.tire;
     // computeTireLoads

@@ -34,7 +34,7 @@ struct K18 {
     double qy1, qy2;  // InterpQ coefficients of dfy0 (2 w1 - w2/2, w1 - w2/2)
 };
 
-__device__ __forceinline__ void k18_init(const PSet& P, double m, double jz, double jx, double dfy0Pn, double dfy0P2n,
+LFDM_HD void k18_init(const PSet& P, double m, double jz, double jx, double dfy0Pn, double dfy0P2n,
                                          double tqs, double maxBrake, K18& k) {
     const double a = P.a, b = P.b, h = P.h, cf = P.cf, cr = P.cr, muf = P.muf, mur = P.mur;
     const double huf = P.tire.r0, hur = P.tire.r0;
@@ -69,7 +69,7 @@ __device__ __forceinline__ void k18_init(const PSet& P, double m, double jz, dou
 // Tire RHS, TMeasy (dof18.cpp:380-539): vertical load from the load transfer, static radius from the
 // tire compression, then the force core.  Off the ground the tire resets and keeps its deflection
 // rates (dof18.cpp:399-409).
-__device__ __forceinline__ void tire_tmeasy(const TireP& t, const Derived& u, const K18& k, double delta, double fz,
+LFDM_HD void tire_tmeasy(const TireP& t, const Derived& u, const K18& k, double delta, double fz,
                                             double vsx, double vsy, double& xe, double& ye, double& xedot,
                                             double& yedot, double& omega, double& fx, double& fy, double& My,
                                             double& xt, double& rStat) {
@@ -83,7 +83,7 @@ __device__ __forceinline__ void tire_tmeasy(const TireP& t, const Derived& u, co
 }
 
 // Tire RHS, TMeasyNr (dof18.cpp:541-673).
-__device__ __forceinline__ void tire_tmeasy_nr(const TireP& t, const Derived& u, const K18& k, double delta, double fz,
+LFDM_HD void tire_tmeasy_nr(const TireP& t, const Derived& u, const K18& k, double delta, double fz,
                                                double vsx, double vsy, double& omega, double& fx, double& fy,
                                                double& My, double& xt, double& rStat) {
     if (fz <= 0) {
@@ -96,9 +96,16 @@ __device__ __forceinline__ void tire_tmeasy_nr(const TireP& t, const Derived& u,
 }
 
 // One half-implicit step (rhsFun order, dof18_halfImplicit.cpp:618-673; update :245-318).
-template <bool NR, bool FULL>
-__device__ __forceinline__ void step18(const PSet& P, const Derived& u, const K18& k, S18& s, double steering,
-                                       double throttle, double braking, double dt, X18* x) {
+// Tire right-hand-side outputs of one step (before the front forces are rotated), captured for the
+// adjoint's reverse sweep.
+struct TireOut18 {
+    double fx[4], fy[4], My[4], rStat[4], xedot[4], yedot[4];
+};
+
+template <bool NR, bool FULL, bool CAPTURE = false>
+LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, double steering,
+                                       double throttle, double braking, double dt, X18* x,
+                                       TireOut18* to = nullptr) {
     const TireP& t = P.tire;
     // computeTireLoads
     const double lat = s.vdot + s.wz * s.u;
@@ -137,6 +144,13 @@ __device__ __forceinline__ void step18(const PSet& P, const Derived& u, const K1
                         fx[i], fy[i], My[i], xt[i], rStat[i]);
         else
             tire_tmeasy_nr(t, u, k, d, fz[i], vsx[i], vsy[i], s.omega[i], fx[i], fy[i], My[i], xt[i], rStat[i]);
+    }
+    if (CAPTURE) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            to->fx[i] = fx[i]; to->fy[i] = fy[i]; to->My[i] = My[i]; to->rStat[i] = rStat[i];
+            to->xedot[i] = s.xedot[i]; to->yedot[i] = s.yedot[i];
+        }
     }
     // computePowertrainRHS
     double dOm[4], eng[4], dcrank;

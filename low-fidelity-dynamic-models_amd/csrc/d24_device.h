@@ -41,7 +41,7 @@ struct K24 {
     double inv_m, inv_jx, inv_jy, inv_jz, mu_g[2], inv_mu[2], half_cf, half_cr;
 };
 
-__device__ __forceinline__ void k24_init(const PSet& P, double m, double jz, double jx, double dfy0Pn, double dfy0P2n,
+LFDM_HD void k24_init(const PSet& P, double m, double jz, double jx, double dfy0Pn, double dfy0P2n,
                                          double tqs, double maxBrake, K24& k) {
     k.m = m; k.tqs = tqs; k.maxBrake = maxBrake;
     k.qy1 = 2. * dfy0Pn - 0.5 * dfy0P2n;
@@ -60,7 +60,7 @@ __device__ __forceinline__ void k24_init(const PSet& P, double m, double jz, dou
 
 // One half-implicit step (rhsFun order, dof24_halfImplicit.cpp:594-706; update :188-297).
 template <bool NR, bool FULL>
-__device__ __forceinline__ void step24(const PSet& P, const Derived& u, const K24& k, S24& s, double steering,
+LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, double steering,
                                        double throttle, double braking, double dt, X24* x) {
     const TireP& t = P.tire;
     const double a = P.a, b = P.b, cf = P.cf, cr = P.cr;

@@ -15,16 +15,20 @@ constexpr double kG = 9.81;                                      // utils.h:13
 constexpr double k2Pi = 6.283185307179586476925286766559;       // utils.h:12
 constexpr double kHalfSqrt2 = 0.70710678118654757;              // sqrt(2.) / 2.
 
+// Host-and-device helpers: the model code below also compiles for the host, where the adjoint's
+// finite-difference checks run it.
+#define LFDM_HD __host__ __device__ __forceinline__
+
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-__device__ __forceinline__ double sgn(double v) { return (double)((0.0 < v) - (v < 0.0)); }
+LFDM_HD double sgn(double v) { return (double)((0.0 < v) - (v < 0.0)); }
 
-__device__ __forceinline__ double clampd(double v, double lo, double hi) {
+LFDM_HD double clampd(double v, double lo, double hi) {
     return v < lo ? lo : (v > hi ? hi : v);
 }
 
 // sineStep (utils.cpp:128-138) with x1, y1, x2, y2 = 1, 1, 1.5, 0: the tire-force weighting.
-__device__ __forceinline__ double sine_step_weight(double x) {
+LFDM_HD double sine_step_weight(double x) {
     if (x <= 1.0) return 1.0;
     if (x >= 1.5) return 0.0;
     const double dx = 0.5, dy = -1.0;
@@ -32,7 +36,7 @@ __device__ __forceinline__ double sine_step_weight(double x) {
 }
 
 // General sineStep (utils.cpp:128-138).
-__device__ __forceinline__ double sine_step(double x, double x1, double y1, double x2, double y2) {
+LFDM_HD double sine_step(double x, double x1, double y1, double x2, double y2) {
     if (x <= x1) return y1;
     if (x >= x2) return y2;
     const double dx = x2 - x1, dy = y2 - y1;
@@ -42,7 +46,7 @@ __device__ __forceinline__ double sine_step(double x, double x1, double y1, doub
 // Clamped piecewise-linear lookup (utils.cpp:109-125). `m` is wave-uniform, so the knot scan
 // runs on scalar loads; the interval index counts knots strictly below x, which is exactly
 // std::lower_bound's answer on a sorted map.
-__device__ __forceinline__ double map_y(const Map& m, double x) {
+LFDM_HD double map_y(const Map& m, double x) {
     const int n = m.n;
     if (x <= m.e[0].x) return m.e[0].y;
     if (x >= m.e[n - 1].x) return m.e[n - 1].y;
@@ -57,7 +61,7 @@ __device__ __forceinline__ double map_y(const Map& m, double x) {
 
 // Same lookup on a map whose abscissae are scaled by sx and ordinates by sy1 then sy2
 // (multiplying by 1.0 is exact, so unit factors reproduce the unscaled map bit for bit).
-__device__ __forceinline__ double map_y_scaled(const Map& m, double x, double sx, double sy1, double sy2) {
+LFDM_HD double map_y_scaled(const Map& m, double x, double sx, double sy1, double sy2) {
     const int n = m.n;
     if (x <= m.e[0].x * sx) return m.e[0].y * sy1 * sy2;
     if (x >= m.e[n - 1].x * sx) return m.e[n - 1].y * sy1 * sy2;
@@ -72,7 +76,7 @@ __device__ __forceinline__ double map_y_scaled(const Map& m, double x, double sx
 }
 
 // Driver-table lookup at a wave-uniform time (utils.cpp:34-56).
-__device__ __forceinline__ void driver_input(double t, const DrvRow* __restrict__ rows, int n, double& s, double& th,
+LFDM_HD void driver_input(double t, const DrvRow* __restrict__ rows, int n, double& s, double& th,
                                              double& b) {
     if (t <= rows[0].t) {
         s = rows[0].steering; th = rows[0].throttle; b = rows[0].braking;
@@ -93,7 +97,7 @@ __device__ __forceinline__ void driver_input(double t, const DrvRow* __restrict_
 }
 
 // Gear-indexed reads by a per-lane gear over a wave-uniform table (select chain, no memory).
-__device__ __forceinline__ double pick(const double* tab, int n, int i) {
+LFDM_HD double pick(const double* tab, int n, int i) {
     double v = tab[0];
     for (int k = 1; k < n; ++k) v = (i == k) ? tab[k] : v;
     return v;

@@ -10,7 +10,7 @@ namespace lfdm {
 namespace dev {
 
 // tmxy_combined (dof18.cpp:247-288); returns fos, and f through `f`.
-__device__ __forceinline__ double tmxy_fos(double s, double df0, double sm, double fm, double ss, double fs,
+LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, double fs,
                                            double& f) {
     double df0loc = 0.0;
     if (sm > 0.0) {
@@ -58,7 +58,7 @@ struct Curve {
 };
 
 // qy1, qy2: InterpQ coefficients of the (per-vehicle) initial lateral slope dfy0.
-__device__ __forceinline__ void curve_at(const TireP& t, const Derived& u, double qy1, double qy2, double fz,
+LFDM_HD void curve_at(const TireP& t, const Derived& u, double qy1, double qy2, double fz,
                                          Curve& c) {
     const double fzn = fz / t.pn;
     const double fzn1 = fzn - 1.;
@@ -79,7 +79,7 @@ __device__ __forceinline__ void curve_at(const TireP& t, const Derived& u, doubl
 // TMeasy force core (dof18.cpp:412-539; the same code is dof11.cpp:335-472 and dof24.cpp:851-987)
 // for a tire on the ground (fz > 0) with static radius rStat.  Returns the deflection rates and
 // the forces; vsx, vsy are the contact-patch velocities of the transform.
-__device__ __forceinline__ void tmeasy_core(const TireP& t, const Derived& u, double qy1, double qy2, double delta,
+LFDM_HD void tmeasy_core(const TireP& t, const Derived& u, double qy1, double qy2, double delta,
                                             double fz, double rStat, double vsx, double vsy, double xe, double ye,
                                             double omega, double& xedot, double& yedot, double& fx, double& fy,
                                             double& My) {
@@ -144,7 +144,7 @@ __device__ __forceinline__ void tmeasy_core(const TireP& t, const Derived& u, do
 
 // TMeasyNr force core (dof18.cpp:571-673 with the Coulomb blend of :290-306; the same code is
 // dof11.cpp:503-606 and dof24.cpp:1014-1121) for a tire on the ground with static radius rStat.
-__device__ __forceinline__ void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double qy2,
+LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double qy2,
                                                double delta, double fz, double rStat, double vsx, double vsy,
                                                double omega, double& fx, double& fy, double& My) {
     const double vsx_in = vsx;
@@ -200,7 +200,7 @@ __device__ __forceinline__ void tmeasy_nr_core(const TireP& t, const Derived& u,
 }
 
 // differentialSplit (dof18.cpp:46-73) with max_bias = 2.
-__device__ __forceinline__ void diff_split(double torque, double sl, double sr, double& tl, double& tr) {
+LFDM_HD void diff_split(double torque, double sl, double sr, double& tl, double& tr) {
     const double max_bias = 2;
     const double diff = fabs(sl - sr);
     double bias = 1;
@@ -218,7 +218,7 @@ __device__ __forceinline__ void diff_split(double torque, double sl, double sr, 
 
 // driveTorque (dof18.cpp:15-42, dof11.cpp:15-42, dof24.cpp:92-119) with the per-vehicle torque
 // scale tqs on the map ordinates.
-__device__ __forceinline__ double drive_torque(const PSet& P, double tqs, double throttle, double w) {
+LFDM_HD double drive_torque(const PSet& P, double tqs, double throttle, double w) {
     if (P.throttleMod) {
         const double tq = map_y_scaled(P.torque, w, throttle, tqs, throttle);
         return tq + map_y(P.losses, w);
@@ -230,7 +230,7 @@ __device__ __forceinline__ double drive_torque(const PSet& P, double tqs, double
 // computePowertrainRHS of the four-wheel models (dof18.cpp:675-845, dof24.cpp:1338-1509): torque
 // converter or slaved crank, gear shift for the next step, axle and differential split, wheel
 // accelerations.  Updates crank (without a converter) and gear; returns dOm, eng and dcrank.
-__device__ __forceinline__ void powertrain4(const PSet& P, const Derived& u, double tqs, double maxBrake,
+LFDM_HD void powertrain4(const PSet& P, const Derived& u, double tqs, double maxBrake,
                                             double throttle, double braking, double vel_u, double& crank, int& gear,
                                             const double* omega, const double* fx, const double* rStat,
                                             const double* My, double* dOm, double* eng, double& dcrank) {

@@ -37,6 +37,33 @@ hipError_t launch_d11(const Launch& L, int block, hipStream_t stream);
 hipError_t launch_d18(const Launch& L, int block, hipStream_t stream);
 hipError_t launch_d24(const Launch& L, int block, hipStream_t stream);
 
+// Differentiation form (18-DOF TMeasy).  Forward: from the record state, nsteps steps with per-step
+// controls ctrl[nsteps][3][n_user]; writes xy[nsteps][2][n_user] (position after each step) and
+// the carried state every K steps into ckpt[ceil(nsteps/K)][32][S].  Backward: from xy_bar (same
+// layout as xy) writes ctrl_bar (layout of ctrl), state_bar[75][n_user] (adjoint of the initial
+// record) and param_bar[7][n_user] (m, jz, jx, dfy0Pn, dfy0P2n, torque scale, maxBrakeTorque);
+// seg is a [K][56][S] work buffer (carried state and tire outputs of every step of a segment).
+struct AdjLaunch {
+    const PSet* sets;
+    const int* wave_set;
+    const int* slot_user;
+    const double* state;
+    const double* ovr;
+    const double* ctrl;
+    double* xy;
+    const double* xy_bar;
+    double* ctrl_bar;
+    double* state_bar;
+    double* param_bar;
+    double* ckpt;
+    double* seg;
+    double dt;
+    int nsteps, K, S, n_user;
+};
+
+hipError_t launch_adj_forward(const AdjLaunch& L, hipStream_t stream);
+hipError_t launch_adj_backward(const AdjLaunch& L, hipStream_t stream);
+
 // Gathers full records of `n` slots from SoA state into out[n][rec].
 hipError_t launch_gather(const double* state, int S, int rec, const int* slots, int n, double* out,
                          hipStream_t stream);

@@ -129,6 +129,13 @@ struct lfdm_solver {
     double* d_rec = nullptr;
     size_t rec_cap = 0;
     std::vector<int> outs_host;  // output steps of the launch in flight (source of an async copy)
+    // differentiation form
+    double* d_ckpt = nullptr;
+    size_t ckpt_cap = 0;
+    double* d_seg = nullptr;
+    size_t seg_cap = 0;
+    int adj_nsteps = 0, adj_K = 0;
+    const double* adj_ctrl = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     double last_ms = 0.0;
@@ -473,6 +480,8 @@ int lfdm_destroy(lfdm_solver* h) {
     dfree(h->d_out_steps);
     dfree(h->d_out);
     dfree(h->d_rec);
+    dfree(h->d_ckpt);
+    dfree(h->d_seg);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
@@ -898,6 +907,80 @@ int lfdm_advance(lfdm_solver* h, int nsteps) {
     if (int rc = launch(h, mode, h->t_now, nsteps, nullptr, 0, 0, 0, steps)) return rc;
     if (mode == kCtrlSteps) h->steps_pos += nsteps;
     for (int k = 0; k < nsteps; ++k) h->t_now += h->step;
+    return LFDM_OK;
+}
+
+namespace {
+
+int adjoint_supported(lfdm_solver* h) {
+    if (h->dof != 18) return fail(LFDM_E_UNSUPPORTED, "the adjoint is available for the 18-DOF model");
+    if (h->tire_type != LFDM_TIRE_TMEASY) return fail(LFDM_E_UNSUPPORTED, "the adjoint needs TMeasy tires");
+    for (const Group& g : h->groups)
+        if (g.pset.nonLinearSteer || g.pset.throttleMod)
+            return fail(LFDM_E_UNSUPPORTED, "the adjoint needs linear steering and no throttleMod");
+    return LFDM_OK;
+}
+
+AdjLaunch adj_launch(lfdm_solver* h) {
+    AdjLaunch L{};
+    L.sets = h->d_sets;
+    L.wave_set = h->d_wave_set;
+    L.slot_user = h->d_slot_user;
+    L.state = h->d_state;
+    L.ovr = h->d_ovr;
+    L.ctrl = h->adj_ctrl;
+    L.ckpt = h->d_ckpt;
+    L.seg = h->d_seg;
+    L.dt = h->step;
+    L.nsteps = h->adj_nsteps;
+    L.K = h->adj_K;
+    L.S = h->S;
+    L.n_user = (int)h->total;
+    return L;
+}
+
+}  // namespace
+
+int lfdm_adjoint_forward(lfdm_solver* h, int nsteps, int checkpoint_every, const double* dctrl, double* dxy) {
+    if (int rc = check(h)) return rc;
+    if (nsteps <= 0 || checkpoint_every <= 0 || !dctrl || !dxy) return fail(LFDM_E_ARG, "bad adjoint arguments");
+    if (int rc = adjoint_supported(h)) return rc;
+    if (int rc = ensure_device(h)) return rc;
+    if (int rc = sync_inputs(h)) return rc;
+    const int K = std::min(checkpoint_every, nsteps);
+    const size_t nseg = (size_t)((nsteps + K - 1) / K);
+    if (int rc = ensure_buffer(h->d_ckpt, h->ckpt_cap, nseg * 32 * (size_t)h->S)) return rc;
+    if (int rc = ensure_buffer(h->d_seg, h->seg_cap, (size_t)K * 56 * (size_t)h->S)) return rc;
+    h->adj_nsteps = nsteps;
+    h->adj_K = K;
+    h->adj_ctrl = dctrl;
+    AdjLaunch L = adj_launch(h);
+    L.xy = dxy;
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    hipError_t e = launch_adj_forward(L, h->stream);
+    if (e != hipSuccess) return fail(LFDM_E_HIP, std::string("adjoint forward launch failed: ") + hipGetErrorString(e));
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    h->timed = true;
+    return LFDM_OK;
+}
+
+int lfdm_adjoint_backward(lfdm_solver* h, const double* dxy_bar, double* dctrl_bar, double* dstate_bar,
+                          double* dparam_bar) {
+    if (int rc = check(h)) return rc;
+    if (!dxy_bar || !dctrl_bar || !dstate_bar || !dparam_bar) return fail(LFDM_E_ARG, "null adjoint buffer");
+    if (h->adj_nsteps <= 0) return fail(LFDM_E_STATE, "adjoint backward before adjoint forward");
+    AdjLaunch L = adj_launch(h);
+    L.xy_bar = dxy_bar;
+    L.ctrl_bar = dctrl_bar;
+    L.state_bar = dstate_bar;
+    L.param_bar = dparam_bar;
+    // fields of the initial-record adjoint that the dynamics never read stay zero
+    HIPCHK(hipMemsetAsync(dstate_bar, 0, (size_t)h->rec * h->total * sizeof(double), h->stream));
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    hipError_t e = launch_adj_backward(L, h->stream);
+    if (e != hipSuccess) return fail(LFDM_E_HIP, std::string("adjoint backward launch failed: ") + hipGetErrorString(e));
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    h->timed = true;
     return LFDM_OK;
 }
 

@@ -38,6 +38,15 @@ if not os.path.exists(LIB_PATH):
         "(or __graft_entry__.build()) first"
     )
 
+# One HIP runtime per process: torch ships its own libamdhip64 / libhsa-runtime64, and a process that
+# loads this library first and torch afterwards ends up with two runtimes whose device memory and
+# streams are not shared (torch tensors handed to the solver's kernels are then invalid).  Importing
+# torch first, when it is installed, makes the library bind to the runtime torch already loaded.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 lib = ctypes.CDLL(LIB_PATH)
 
 _p = ctypes.c_void_p
@@ -83,6 +92,8 @@ SIGNATURES = {
     "lfdm_initialized_count": [_p, ctypes.POINTER(_u)],
     "lfdm_advance": [_p, _i],
     "lfdm_set_time": [_p, _d],
+    "lfdm_adjoint_forward": [_p, _i, _i, _p, _p],
+    "lfdm_adjoint_backward": [_p, _p, _p, _p, _p],
     "lfdm_set_stream": [_p, _p],
     "lfdm_synchronize": [_p],
     "lfdm_last_launch_ms": [_p, _pd],

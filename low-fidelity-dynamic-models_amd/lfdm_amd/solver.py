@@ -236,6 +236,15 @@ class SolverHalfImplicitGPU:
     def set_time(self, t):
         check(lib.lfdm_set_time(self._h, t))
 
+    # -- differentiation form (device pointers, e.g. torch tensors' data_ptr()) ---------------
+    def adjoint_forward(self, nsteps, checkpoint_every, ctrl_ptr, xy_ptr):
+        check(lib.lfdm_adjoint_forward(self._h, nsteps, checkpoint_every, ctypes.c_void_p(ctrl_ptr),
+                                       ctypes.c_void_p(xy_ptr)))
+
+    def adjoint_backward(self, xy_bar_ptr, ctrl_bar_ptr, state_bar_ptr, param_bar_ptr):
+        check(lib.lfdm_adjoint_backward(self._h, ctypes.c_void_p(xy_bar_ptr), ctypes.c_void_p(ctrl_bar_ptr),
+                                        ctypes.c_void_p(state_bar_ptr), ctypes.c_void_p(param_bar_ptr)))
+
     def set_stream(self, stream_ptr):
         check(lib.lfdm_set_stream(self._h, ctypes.c_void_p(stream_ptr)))
 

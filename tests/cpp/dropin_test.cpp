@@ -24,9 +24,11 @@ void expect_near(const char* what, double expected, double got, double tol) {
     if (!ok) ++failures;
 }
 
+// Throttle from t = 0.5 s, a kernel-sim-time boundary (chunks of 0.1 s start at 0, 0.1, ... 0.4,
+// 0.5 exactly), so the per-step functor and the per-chunk scalar SolveStep see the same controls.
 struct MyFunctor {
     __device__ void operator()(double t, DriverInput* controls) {
-        if (t < 1) {
+        if (t < 0.5) {
             controls->m_steering = 0.0;
             controls->m_throttle = 0.0;
             controls->m_braking = 0.0;
@@ -125,7 +127,7 @@ int main(int argc, char** argv) {
             ref.SetKernelSimTime(0.1);
             ref.Initialize(veh_st, tirelf_st, tirerf_st, tirelr_st, tirerr_st, 64);
             double t2 = 0.;
-            while (t2 < 10.0 - 1e-4) t2 = ref.SolveStep(t2, 0.0, t2 < 1 ? 0.0 : 0.5, 0.0);
+            while (t2 < 10.0 - 1e-4) t2 = ref.SolveStep(t2, 0.0, t2 < 0.5 ? 0.0 : 0.5, 0.0);
             const double a = solver.GetSimState(63)._veh_state._x, b = ref.GetSimState(63)._veh_state._x;
             expect_near("functor SolveStep equals scalar SolveStep (x)", b, a, 0.0);
             expect_near("functor SolveStep moved the vehicle", 1.0, a > 10.0 ? 1.0 : 0.0, 0.0);

@@ -1,0 +1,177 @@
+"""Differentiation form on the GPU: the hand-written reverse-mode adjoint of the 18-DOF step
+(csrc/d18_adjoint.h, kernels in csrc/d18_adjoint.hip) against fourth-order central finite
+differences of the GPU forward map, through the C ABI and through torch.autograd.
+
+Bar: for every input group (controls, per-vehicle parameters scaled by their values, initial state)
+max |adjoint - fd| <= 1e-6 max |fd| (normwise relative error of the gradient); 1e-4 for the tire
+deflections xe, ye, whose finite differences stop converging near 1e-5 (they are ~1e-5 m and stiff:
+steps small enough to stay off the slip-force kinks come within a decade of the rounding noise;
+the host check of the same code, tests/test_adjoint_host.py, reaches 8e-6).  FD uses (J(-2h) - 8 J(-h) + 8 J(h) - J(2h)) / 12h on a fixed random linear functional J
+of the (x, y) trajectory, all perturbed copies of a vehicle integrated in one batch; steps: 1e-4 for
+controls, 3e-6 relative for parameters, 1e-8 for deflections, 1e-7 for roll angle and rate, 1e-5
+(relative above 1) for the other states.  The same check runs on the CPU over the kernels' own lane
+code (tests/test_adjoint_host.py).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import lfdm_amd
+from lfdm_amd import workloads as W
+from conftest import DATA
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def hmmwv():
+    d = f"{DATA}/18dof/HMMWV"
+    return lfdm_amd.params_from_json(18, f"{d}/vehicle.json", f"{d}/tmeasy.json")
+
+
+def test_adjoint_matches_fourth_order_fd():
+    from lfdm_amd.mpc import moving_start
+
+    flat = hmmwv()
+    fields = lfdm_amd.record_fields(18)
+    rng = np.random.default_rng(3)
+    nb, T, K = 3, 600, 64  # 10 segments, the last one partial
+    dev = "cuda"
+    # base vehicles: jittered parameters, moving starts, smooth random controls with braking
+    base_p = W.default_overrides(flat, nb)
+    base_p[0] *= rng.uniform(0.9, 1.1, nb)
+    base_p[1] *= rng.uniform(0.9, 1.1, nb)
+    base_p[3:5] *= rng.uniform(0.9, 1.1, nb)
+    base_i = moving_start(flat, nb, speed=12.0).numpy()
+    base_i[:, fields.index("v")] = 0.05
+    base_i[:, fields.index("wz")] = 0.02
+    t = np.arange(T) * 1e-3
+    base_c = np.empty((T, 3, nb))
+    for b in range(nb):
+        base_c[:, 0, b] = 0.2 * np.sin(2.0 * t + b) + 0.05
+        base_c[:, 1, b] = 0.35 + 0.15 * np.sin(3.0 * t + b)
+        base_c[:, 2, b] = np.where(t > 0.4, 0.08, 0.0)
+    w = rng.uniform(-1, 1, (T, 2, nb))
+    # directions
+    ctrl_dirs = [(int(j), int(q)) for j, q in zip(rng.integers(0, T, 8), rng.integers(0, 3, 8))] + [(T - 1, 0), (0, 1)]
+    state_dirs = ["x", "y", "u", "v", "psi", "wz", "phi", "wx", "udot", "vdot", "crankOmega", "lf.omega", "rr.omega",
+                  "lf.xe", "rf.ye", "lr.xe", "rr.ye"]
+    dirs = [("c", d) for d in ctrl_dirs] + [("p", i) for i in range(7)] + [("s", f) for f in state_dirs]
+    offs = (-2, -1, 1, 2)
+    n = nb * (1 + 4 * len(dirs))
+    P = np.repeat(base_p, 1 + 4 * len(dirs), axis=1)
+    I = np.repeat(base_i, 1 + 4 * len(dirs), axis=0)
+    C = np.repeat(base_c, 1 + 4 * len(dirs), axis=2)
+    hs = []
+    for b in range(nb):
+        col0 = b * (1 + 4 * len(dirs))
+        for di, (kind, d) in enumerate(dirs):
+            if kind == "c":
+                h = 1e-4
+            elif kind == "p":
+                h = 3e-6 * abs(base_p[d, b])
+            else:
+                x0 = base_i[b, fields.index(d)]
+                h = (1e-8 if d.endswith((".xe", ".ye")) else 1e-7 if d in ("phi", "wx") else 1e-5) * max(1.0, abs(x0))
+            hs.append(h)
+            for oi, o in enumerate(offs):
+                col = col0 + 1 + 4 * di + oi
+                if kind == "c":
+                    C[d[0], d[1], col] += o * h
+                elif kind == "p":
+                    P[d, col] += o * h
+                else:
+                    I[col, fields.index(d)] += o * h
+    s = lfdm_amd.SolverHalfImplicitGPU(18, n)
+    s.construct_flat(flat, n, None, lfdm_amd.TMeasy)
+    s.set_overrides(P)
+    s.initialize_batch(I)
+    ctrl = torch.from_numpy(np.ascontiguousarray(C)).to(dev)
+    xy = torch.empty((T, 2, n), dtype=torch.float64, device=dev)
+    say(f"[adjoint] forward of {n} vehicles x {T} steps (checkpoint every {K})")
+    s.adjoint_forward(T, K, ctrl.data_ptr(), xy.data_ptr())
+    s.synchronize()
+    W_all = np.repeat(w, 1 + 4 * len(dirs), axis=2)
+    J = (xy.cpu().numpy() * W_all).sum(axis=(0, 1))
+    xyb = torch.from_numpy(np.ascontiguousarray(W_all)).to(dev)
+    gc = torch.empty_like(ctrl)
+    gs = torch.empty((len(fields), n), dtype=torch.float64, device=dev)
+    gp = torch.empty((7, n), dtype=torch.float64, device=dev)
+    say("[adjoint] backward")
+    s.adjoint_backward(xyb.data_ptr(), gc.data_ptr(), gs.data_ptr(), gp.data_ptr())
+    s.synchronize()
+    gc, gs, gp = gc.cpu().numpy(), gs.cpu().numpy(), gp.cpu().numpy()
+    worst = {}
+    k = 0
+    for b in range(nb):
+        col0 = b * (1 + 4 * len(dirs))
+        err = {"c": [0.0, 0.0], "p": [0.0, 0.0], "s": [0.0, 0.0], "d": [0.0, 0.0]}
+        for di, (kind, d) in enumerate(dirs):
+            h = hs[k]
+            k += 1
+            Jd = J[col0 + 1 + 4 * di: col0 + 5 + 4 * di]
+            fd = (Jd[0] - 8 * Jd[1] + 8 * Jd[2] - Jd[3]) / (12 * h)
+            if kind == "c":
+                ad, unit = gc[d[0], d[1], col0], 1.0
+            elif kind == "p":
+                ad, unit = gp[d, col0], abs(base_p[d, b])
+            else:
+                ad, unit = gs[fields.index(d), col0], 1.0
+            g = "d" if kind == "s" and d.endswith((".xe", ".ye")) else kind
+            err[g][0] = max(err[g][0], abs(ad - fd) * unit)
+            err[g][1] = max(err[g][1], abs(fd) * unit)
+        for kind, (num, den) in err.items():
+            worst[kind] = max(worst.get(kind, 0.0), num / den)
+    say(f"[adjoint] normwise relative error vs 4th-order FD: controls {worst['c']:.2e}, parameters "
+        f"{worst['p']:.2e}, initial state {worst['s']:.2e}, tire deflections {worst['d']:.2e}")
+    assert max(worst["c"], worst["p"], worst["s"]) <= 1e-6
+    assert worst["d"] <= 1e-4
+    # identical vehicles get identical gradients; untouched record fields get none
+    assert np.all(gs[fields.index("lf.fx"), :] == 0.0)
+    s.close()
+
+
+def test_torch_rollout_backward_and_mpc_step():
+    """rollout() is a torch.autograd.Function: loss.backward() fills controls.grad, params.grad and
+    init.grad; gradients are linear in grad_output; a few MPC iterations reduce the tracking loss."""
+    from lfdm_amd.mpc import arc_reference, initial_controls, moving_start, run_mpc, tracking_loss
+    from lfdm_amd.torch_ops import default_params, make_solver, rollout
+
+    flat = hmmwv()
+    n, T = 256, 1500
+    s = make_solver(flat, n)
+    ref, kappa = arc_reference(n, T, seed=4)
+    init = moving_start(flat, n)
+    params = default_params(flat, n)
+    c0 = initial_controls(flat, n, T, kappa)
+    ctrl = c0.clone().requires_grad_(True)
+    p = params.clone().requires_grad_(True)
+    i0 = init.clone().requires_grad_(True)
+    xy = rollout(s, ctrl, p, i0, 100)
+    loss = tracking_loss(xy, ref).sum()
+    loss.backward()
+    assert ctrl.grad is not None and p.grad is not None and i0.grad is not None
+    assert torch.isfinite(ctrl.grad).all() and torch.isfinite(p.grad).all()
+    g1 = ctrl.grad.clone()
+    # scaling the loss scales the gradient exactly (the backward is linear in grad_output)
+    ctrl.grad = None
+    xy = rollout(s, ctrl, p, i0, 100)
+    (2.0 * tracking_loss(xy, ref).sum()).backward()
+    assert torch.equal(ctrl.grad, 2.0 * g1)
+    # checkpoint interval does not change the result
+    ctrl.grad = None
+    xy = rollout(s, ctrl, p, i0, 37)
+    tracking_loss(xy, ref).sum().backward()
+    assert torch.equal(ctrl.grad, g1)
+    say("[adjoint] MPC, 6 iterations")
+    _, hist = run_mpc(s, flat, init, params, ref, c0, iterations=6, lr=0.05)
+    say(f"[adjoint] MPC mean loss per vehicle: {[round(h, 4) for h in hist]}")
+    assert hist[-1] < hist[0]
+    s.close()
