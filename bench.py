@@ -132,10 +132,105 @@ def adjoint_cost(flat, n=65536, T=5000, K=100, mpc_iterations=20):
     return {"vehicles": n, "steps": T, "checkpoint_every": K, "forward_ms": f_ms, "backward_ms": b_ms,
             "ratio": (f_ms + b_ms) / f_ms, "plain_forward_ms": p_ms, "ratio_vs_plain_forward": (f_ms + b_ms) / p_ms,
             "central_fd_cost_ratio": 2.0 * n_inputs * p_ms / (f_ms + b_ms),
-            "mpc": {"iterations": mpc_iterations, "seconds": mpc_s, "mean_loss_first": hist[0],
-                    "mean_loss_last": hist[-1]},
+            "mpc": {"iterations": mpc_iterations, "seconds": mpc_s, "mean_loss_first": hist[0] if hist else None,
+                    "mean_loss_last": hist[-1] if hist else None},
             "note": "ratio = (adjoint forward + backward) / adjoint forward device time; central_fd_cost_ratio = "
                     "cost of central differences (2 plain forwards per input) / adjoint cost"}
+
+
+def mixed_fleet(n_total=2_000_000, split=(0.4, 0.4, 0.2), tmin=2.0, tmax=20.0, jitter=0.15, trial_steps=1000,
+                reps=2, seed=11):
+    """Batching form on the mixed workload: n_total vehicles split 40/40/20 over the 11-, 18- and
+    24-DOF HMMWV models, per-vehicle horizons T_i ~ U(tmin, tmax) s (steps_i = round(T_i / dt)),
+    mass / yaw inertia / cornering stiffness jittered x U(0.85, 1.15), knot-program controls over
+    tmax; 11/18-DOF vehicles start at rest, 24-DOF ones at U(0, 20) m/s.  One bucket per model, the
+    autotuner picks layout / block size / steps per launch, then each timed run is one graph
+    launch over the longest horizon (records reset between runs, outside the timed region).
+    Throughput counts only the steps each vehicle takes (sum of its horizon steps).  For
+    comparison: the same fleet in Construct order with one eager launch per model in turn."""
+    import lfdm_amd
+    from lfdm_amd import LAYOUT_GROUPED, Fleet, SolverHalfImplicitGPU, TMeasy
+    from lfdm_amd import workloads as W
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dt = 1e-3
+    buckets, inits, horizons = [], [], []
+    for k, (dof, frac) in enumerate(zip((11, 18, 24), split)):
+        n = int(round(n_total * frac))
+        d = os.path.join(W.DATA, f"{dof}dof", "HMMWV")
+        flat = lfdm_amd.params_from_json(dof, f"{d}/vehicle.json", f"{d}/tmeasy.json",
+                                         f"{d}/suspension.json" if dof == 24 else None)
+        fl = W.knot_fleet(n, flat, seed=seed + k, horizon=tmax, jitter=jitter, dof=dof)
+        s = SolverHalfImplicitGPU(dof, n)
+        s.construct_flat(flat, n, None, TMeasy)
+        s.set_overrides(fl["ovr"])
+        if dof == 24:
+            s.initialize_batch(W.initial_speed_records(n, flat, dof, seed=seed + k))
+        else:
+            s.Initialize(s.zero_record(), n)
+        s.set_controls_knots(fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
+        del fl
+        hz = np.rint(rng.uniform(tmin, tmax, n) / dt).astype(np.int32)
+        s.set_horizons(hz)
+        buckets.append(s)
+        inits.append(s.get_states())
+        horizons.append(hz)
+    T = int(max(h.max() for h in horizons))
+    work = float(sum(h.astype(np.float64).sum() for h in horizons))
+
+    def lane_efficiency(hz, sort):
+        h = np.sort(hz)[::-1] if sort else hz
+        pad = (-len(h)) % 64
+        w = np.concatenate([h, np.zeros(pad, h.dtype)]).reshape(-1, 64).max(axis=1)
+        return float(h.sum() / (64.0 * w.sum()))
+
+    def reset():
+        for s, i0 in zip(buckets, inits):
+            s.set_states(i0)
+            s.synchronize()
+
+    f = Fleet(buckets)
+    t0 = time.perf_counter()
+    cfg = f.autotune(T, trial_steps=trial_steps)
+    tune_s = time.perf_counter() - t0
+    reset()
+    f.run(T)  # captures the graph
+    f.synchronize()
+    ms, walls = [], []
+    for _ in range(reps):
+        reset()
+        t0 = time.perf_counter()
+        f.run(T)
+        f.synchronize()
+        walls.append(time.perf_counter() - t0)
+        ms.append(f.last_ms())
+    wall = min(walls)
+    # reference-style comparison: Construct order, one eager launch per model after the other
+    for s in buckets:
+        s.set_layout(LAYOUT_GROUPED)
+    reset()
+    t0 = time.perf_counter()
+    for s in buckets:
+        s.set_time(0.0)
+        s.advance(T)
+    for s in buckets:
+        s.synchronize()
+    eager_wall = time.perf_counter() - t0
+    f.close()
+    for s in buckets:
+        s.close()
+    return {"workload": f"mixed: {n_total:,} vehicles 11/18/24-DOF {split[0]:.0%}/{split[1]:.0%}/{split[2]:.0%}, "
+                        f"horizons U({tmin:g}, {tmax:g}) s at dt=1e-3, +-{jitter:.0%} parameters, knot controls",
+            "vehicle_steps": work, "longest_horizon_steps": T,
+            "value": work / wall, "unit": "vehicle-steps/s", "graph_ms": float(min(ms)), "wall_s": wall,
+            "autotune": {"seconds": tune_s, "trial_steps": trial_steps,
+                         "buckets": [{"dof": d_, "vehicles": int(len(h)), "block": c[0], "steps_per_launch": c[1],
+                                      "layout": "by_horizon" if c[2] else "grouped"}
+                                     for d_, h, c in zip((11, 18, 24), horizons, cfg)]},
+            "lane_efficiency": {"by_horizon": [lane_efficiency(h, True) for h in horizons],
+                                "grouped": [lane_efficiency(h, False) for h in horizons]},
+            "grouped_eager": {"value": work / eager_wall, "wall_s": eager_wall},
+            "speedup_vs_grouped_eager": eager_wall / wall}
 
 
 def measured_traffic_per_step():
@@ -176,6 +271,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation-form measurement")
     ap.add_argument("--adjoint-only", action="store_true", help="only the differentiation-form measurement")
+    ap.add_argument("--mpc-iterations", type=int, default=20, help="MPC iterations of the adjoint measurement")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the batching-form (mixed fleet) measurement")
+    ap.add_argument("--mixed-only", action="store_true", help="only the batching-form measurement")
+    ap.add_argument("--mixed-vehicles", type=int, default=2_000_000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (reads 1 GiB, writes 1 GiB), the "
                          "reference kernel that scales FETCH_SIZE / WRITE_SIZE counter readings to bytes")
@@ -199,9 +298,14 @@ def main():
     from lfdm_amd import SolverHalfImplicitGPU, TMeasy
     from lfdm_amd import workloads as W
 
+    if args.mixed_only:
+        print(json.dumps(mixed_fleet(args.mixed_vehicles)))
+        return
+
     if args.adjoint_only:
         d = os.path.join(W.DATA, "18dof", "HMMWV")
-        print(json.dumps(adjoint_cost(lfdm_amd.params_from_json(18, f"{d}/vehicle.json", f"{d}/tmeasy.json"))))
+        print(json.dumps(adjoint_cost(lfdm_amd.params_from_json(18, f"{d}/vehicle.json", f"{d}/tmeasy.json"),
+                                      mpc_iterations=args.mpc_iterations)))
         return
 
     n, H, dof = args.vehicles, args.horizon_steps, args.dof
@@ -254,9 +358,13 @@ def main():
                "single_core": rate1, "cpu": cpu_model()}
     metric = ("18-DOF FP64 vehicle-steps/s at 1M vehicles; adjoint/forward cost ratio" if dof == 18 else
               f"{dof}-DOF FP64 vehicle-steps/s")
+    s.close()
     adj = None
     if dof == 18 and not args.no_adjoint:
-        adj = adjoint_cost(flat)
+        adj = adjoint_cost(flat, mpc_iterations=args.mpc_iterations)
+    mixed = None
+    if dof == 18 and not args.no_mixed:
+        mixed = mixed_fleet(args.mixed_vehicles)
     line = {"metric": metric, "value": value,
             "unit": "vehicle-steps/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -267,9 +375,8 @@ def main():
                                    (", start at U(0,20) m/s" if dof == 24 else ""),
                        "vehicles": n, "horizon_steps": H, "dt": 1e-3, "threads_per_block": args.tpb},
             "roofline": roofline, "cpu_baseline": cpu, "adjoint": adj,
-            "adjoint_forward_cost_ratio": None if adj is None else adj["ratio"]}
+            "adjoint_forward_cost_ratio": None if adj is None else adj["ratio"], "mixed_fleet": mixed}
     print(json.dumps(line))
-    s.close()
 
 
 if __name__ == "__main__":

@@ -165,6 +165,55 @@ int lfdm_adjoint_forward(lfdm_solver* h, int nsteps, int checkpoint_every, const
                          double* device_xy);
 int lfdm_adjoint_backward(lfdm_solver* h, const double* device_xy_bar, double* device_ctrl_bar,
                           double* device_state_bar, double* device_param_bar);
+/* Device memory the backward may use for its tape (default 16 GiB).  The backward walks the
+ * checkpoint segments in groups, recomputing a whole group's steps at once (56 doubles per vehicle
+ * and step); a group holds as many segments as fill the GPU, within this budget, and at least one.
+ * Results do not depend on it. */
+int lfdm_adjoint_set_tape_budget(lfdm_solver* h, unsigned long long bytes);
+
+/* ---- batching form: per-vehicle horizons, layouts, mixed fleets (no counterpart in the
+ * reference, whose solver has one end time for every vehicle and overshoots it to a whole number of
+ * kernel-sim-time chunks, dof18_halfImplicit_gpu.cu:438-464) -------------------------------------- */
+
+#define LFDM_LAYOUT_GROUPED 0    /* slots in Construct order (the default) */
+#define LFDM_LAYOUT_BY_HORIZON 1 /* within each Construct group, longest horizon first */
+
+/* Per-vehicle horizons steps[total]: vehicle i stops after steps[i] steps counted from t = 0 (a launch
+ * starting at time t0 starts at step round(t0 / step)); later steps leave its record unchanged and
+ * output rows after it repeat its final state at the row's time.  NULL removes the horizons. */
+int lfdm_set_horizons(lfdm_solver* h, const int* steps);
+/* Slot layout (results do not depend on it).  Grouping vehicles of similar horizons into the same
+ * wavefronts lets a wavefront stop when its vehicles do; changing it re-lays the device image out
+ * (the current records are kept). */
+int lfdm_set_layout(lfdm_solver* h, int layout);
+int lfdm_get_layout(lfdm_solver* h, int* layout);
+
+/* A mixed fleet: solver handles of any models (buckets, normally one per model, so a wavefront never
+ * mixes models) integrated together.  lfdm_fleet_run(f, nsteps) does for every bucket what Solve
+ * does without output: nsteps steps from t = 0 from the current records with each bucket's control
+ * source (driver tables, knots or per-step arrays from row 0) and horizons, as one HIP graph with a
+ * branch per bucket, each a chain of launches of steps_per_launch steps (0 = one launch).  The graph
+ * is captured on the first run and replayed while the launch configuration is unchanged.  Runs are
+ * asynchronous on the fleet's stream; the buckets' streams are ordered after them.  The handles
+ * stay owned by the caller and must outlive the fleet. */
+typedef struct lfdm_fleet lfdm_fleet;
+int lfdm_fleet_create(lfdm_fleet** out);
+int lfdm_fleet_destroy(lfdm_fleet* f);
+/* Adds a bucket; returns its index (>= 0) or a negative LFDM_E_*. */
+int lfdm_fleet_add(lfdm_fleet* f, lfdm_solver* h);
+/* Stream for the fleet's runs (hipStream_t; NULL = the fleet's own). */
+int lfdm_fleet_set_stream(lfdm_fleet* f, void* stream);
+/* Launch configuration of a bucket: threads per block (64..256, a multiple of 64) and steps per launch. */
+int lfdm_fleet_set_config(lfdm_fleet* f, int bucket, int block, int steps_per_launch);
+int lfdm_fleet_get_config(lfdm_fleet* f, int bucket, int* block, int* steps_per_launch, int* layout);
+/* Picks every bucket's layout (from the lane-steps each layout would run over nsteps steps with the
+ * bucket's horizons) and its block size and steps per launch (timed trials of min(trial_steps,
+ * nsteps) steps from the current records, which are restored afterwards). */
+int lfdm_fleet_autotune(lfdm_fleet* f, int nsteps, int trial_steps);
+int lfdm_fleet_run(lfdm_fleet* f, int nsteps);
+int lfdm_fleet_synchronize(lfdm_fleet* f);
+/* Device time of the last lfdm_fleet_run, from HIP events around the graph launch. */
+int lfdm_fleet_last_ms(lfdm_fleet* f, double* ms);
 
 /* Stream used for every launch and copy (hipStream_t; NULL = the handle's own stream). */
 int lfdm_set_stream(lfdm_solver* h, void* stream);

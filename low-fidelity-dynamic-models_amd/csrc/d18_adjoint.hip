@@ -1,7 +1,14 @@
-// Differentiation form of the 18-DOF solver: a forward kernel that integrates with per-step
-// controls, records the (x, y) trajectory and stores the carried state every K steps, and a
-// reverse kernel that walks the segments backwards (recompute the segment's states from its
-// checkpoint into a per-vehicle buffer, then run the step adjoint of d18_adjoint.h step by step).
+// Differentiation form of the 18-DOF solver.
+//  - adj_forward: integrates with per-step controls, records the (x, y) trajectory and stores the
+//    carried state every K steps.
+//  - adj_recompute: one lane per (segment of a group, vehicle slot); reruns the segment from its
+//    checkpoint into the tape.  Segments are independent, so a group of G segments gives G times
+//    the wavefronts of the fleet and the recompute runs at full occupancy even for fleets that fill
+//    only one wavefront per SIMD.
+//  - adj_reverse: one lane per slot; walks the group's segments backwards through the step adjoint
+//    of d18_adjoint.h.  The step adjoint needs the whole register file of a SIMD, so the per-vehicle
+//    constants and their adjoints live in LDS (23 + 21 doubles per lane; both structs have an odd
+//    number of doubles, so a wavefront's 64-bit LDS accesses at that stride hit distinct banks).
 //
 // One vehicle per lane as in the integration kernels; every lane of a wave runs the same step
 // counts, so the segment loops are wave-uniform.
@@ -9,43 +16,63 @@
 
 #include "d18_adjoint_lanes.h"
 
-// Waves per SIMD of the reverse kernel (its step adjoint needs far more registers than the forward).
-#ifndef LFDM_ADJ_BWD_WAVES
-#define LFDM_ADJ_BWD_WAVES 1
-#endif
-
 namespace lfdm {
 
 using namespace dev;
 
 namespace {
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void adj_forward(AdjLaunch L) {
+constexpr int kThreads = 256;
+
+static_assert(sizeof(K18) % 16 == 8 && sizeof(K18b) % 16 == 8, "odd number of doubles per lane in LDS");
+
+template <int W>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W, W))) void adj_forward(AdjLaunch L) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int wave = uniform(slot >> 6);
     if (wave * kWave >= L.S) return;
     adj_forward_lane(L, slot, L.sets[uniform(L.wave_set[wave])]);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LFDM_ADJ_BWD_WAVES, LFDM_ADJ_BWD_WAVES))) void
-adj_backward(AdjLaunch L) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void adj_recompute(AdjLaunch L) {
+    const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // S is a multiple of the wavefront size, so q and the slot's wave are wave-uniform
+    const int q = uniform((int)(id / (size_t)L.S));
+    const int slot = (int)(id - (size_t)q * L.S);
+    if (q >= L.seg_hi - L.seg_lo) return;
+    adj_recompute_lane(L, q, slot, L.sets[uniform(L.wave_set[slot >> 6])]);
+}
+
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void adj_reverse(AdjLaunch L) {
+    __shared__ K18 ks[kThreads];
+    __shared__ K18b kbs[kThreads];
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int wave = uniform(slot >> 6);
     if (wave * kWave >= L.S) return;
-    adj_backward_lane(L, slot, L.sets[uniform(L.wave_set[wave])]);
+    adj_reverse_lane(L, slot, L.sets[uniform(L.wave_set[wave])], ks[threadIdx.x], kbs[threadIdx.x]);
 }
 
 }  // namespace
 
-hipError_t launch_adj_forward(const AdjLaunch& L, hipStream_t stream) {
+hipError_t launch_adj_forward(const AdjLaunch& L, int waves_per_simd, hipStream_t stream) {
     if (L.S <= 0 || L.nsteps <= 0) return hipSuccess;
-    hipLaunchKernelGGL(adj_forward, (L.S + 255) / 256, 256, 0, stream, L);
+    if (waves_per_simd == 1)
+        hipLaunchKernelGGL(adj_forward<1>, (L.S + kThreads - 1) / kThreads, kThreads, 0, stream, L);
+    else
+        hipLaunchKernelGGL(adj_forward<2>, (L.S + kThreads - 1) / kThreads, kThreads, 0, stream, L);
     return hipGetLastError();
 }
 
-hipError_t launch_adj_backward(const AdjLaunch& L, hipStream_t stream) {
-    if (L.S <= 0 || L.nsteps <= 0) return hipSuccess;
-    hipLaunchKernelGGL(adj_backward, (L.S + 255) / 256, 256, 0, stream, L);
+hipError_t launch_adj_recompute(const AdjLaunch& L, hipStream_t stream) {
+    const size_t lanes = (size_t)(L.seg_hi - L.seg_lo) * (size_t)L.S;
+    if (L.S <= 0 || lanes == 0) return hipSuccess;
+    hipLaunchKernelGGL(adj_recompute, (unsigned)((lanes + kThreads - 1) / kThreads), kThreads, 0, stream, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_reverse(const AdjLaunch& L, hipStream_t stream) {
+    if (L.S <= 0 || L.seg_hi <= L.seg_lo) return hipSuccess;
+    hipLaunchKernelGGL(adj_reverse, (L.S + kThreads - 1) / kThreads, kThreads, 0, stream, L);
     return hipGetLastError();
 }
 

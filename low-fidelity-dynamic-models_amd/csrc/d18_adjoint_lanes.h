@@ -103,32 +103,104 @@ LFDM_HD void adj_forward_lane(const AdjLaunch& L, int slot, const PSet& P) {
     }
 }
 
-LFDM_HD void adj_backward_lane(const AdjLaunch& L, int slot, const PSet& P) {
+// Adjoint of the carried state and of the per-vehicle constants, kept between the reverse launches
+// of successive segment groups: carry[kCarry][S].
+constexpr int kCarry = 31 + 21;
+static_assert(kCarried == kAdjCkptRow && kSegRow == kAdjTapeRow && kCarry == kAdjCarry, "adjoint buffer rows");
+
+LFDM_HD void store_carry(const S18b& l, const K18b& kb, double* p, size_t S) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        p[(0 + i) * S] = l.xe[i];
+        p[(4 + i) * S] = l.ye[i];
+        p[(8 + i) * S] = l.xedot[i];
+        p[(12 + i) * S] = l.yedot[i];
+        p[(16 + i) * S] = l.omega[i];
+    }
+    const double v[32] = {l.x,      l.y,       l.u,     l.v,         l.psi,    l.wz,    l.phi,    l.wx,
+                          l.udot,   l.vdot,    l.crank, kb.Z1f,      kb.Z2fc,  kb.Z1r,  kb.Z2rc,  kb.Z4n,
+                          kb.neg_mt, kb.E3c,   kb.hm,   kb.A2,       kb.A3,    kb.inv_mt, kb.two_hrc_m, kb.denom,
+                          kb.A1A2,  kb.A1A3,   kb.A3sq, kb.qy1,      kb.qy2,   kb.jz,   kb.tqs,   kb.maxBrake};
+#pragma unroll
+    for (int j = 0; j < 32; ++j) p[(20 + j) * S] = v[j];
+}
+
+LFDM_HD void load_carry(const double* p, size_t S, S18b& l, K18b& kb) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        l.xe[i] = p[(0 + i) * S];
+        l.ye[i] = p[(4 + i) * S];
+        l.xedot[i] = p[(8 + i) * S];
+        l.yedot[i] = p[(12 + i) * S];
+        l.omega[i] = p[(16 + i) * S];
+    }
+    double v[32];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) v[j] = p[(20 + j) * S];
+    l.x = v[0]; l.y = v[1]; l.u = v[2]; l.v = v[3]; l.psi = v[4]; l.wz = v[5]; l.phi = v[6]; l.wx = v[7];
+    l.udot = v[8]; l.vdot = v[9]; l.crank = v[10];
+    kb.Z1f = v[11]; kb.Z2fc = v[12]; kb.Z1r = v[13]; kb.Z2rc = v[14]; kb.Z4n = v[15]; kb.neg_mt = v[16];
+    kb.E3c = v[17]; kb.hm = v[18]; kb.A2 = v[19]; kb.A3 = v[20]; kb.inv_mt = v[21]; kb.two_hrc_m = v[22];
+    kb.denom = v[23]; kb.A1A2 = v[24]; kb.A1A3 = v[25]; kb.A3sq = v[26]; kb.qy1 = v[27]; kb.qy2 = v[28];
+    kb.jz = v[29]; kb.tqs = v[30]; kb.maxBrake = v[31];
+}
+
+// Keeps the compiler from carrying values read from memory across this point in registers: the
+// reverse lane keeps the per-vehicle constants and their adjoints in memory (LDS on the GPU) and
+// reads them where each step uses them, which frees ~90 registers of the reverse step.
+#define LFDM_RELOAD_MEMORY() asm volatile("" ::: "memory")
+
+// Segment recompute of the backward pass, one lane per (segment seg_lo + q, slot): from the
+// segment's checkpoint, reruns its steps and writes each step's carried state and tire outputs to
+// tape block q ([q][K][kSegRow][S]).  Segments are independent, so a launch covers a whole group.
+LFDM_HD void adj_recompute_lane(const AdjLaunch& L, int q, int slot, const PSet& P) {
+    const int seg = L.seg_lo + q;
+    if (seg >= L.seg_hi) return;
     const size_t S = (size_t)L.S, N = (size_t)L.n_user;
     K18 k;
+    load_k(P, L.ovr + slot, S, k);
+    S18 s;
+    unpack(L.ckpt + (size_t)seg * kCarried * S + slot, S, s);
+    const int user = L.slot_user[slot];
+    const int uv = user >= 0 ? user : 0;
+    const int j0 = seg * L.K;
+    const int j1 = (L.nsteps < j0 + L.K ? L.nsteps : j0 + L.K);
+    double* blk = L.tape + (size_t)q * L.K * kSegRow * S + slot;
+    for (int j = j0; j < j1; ++j) {
+        double* row = blk + (size_t)(j - j0) * kSegRow * S;
+        pack(s, row, S);
+        const double* c = L.ctrl + (size_t)j * 3 * N + uv;
+        TireOut18 to;
+        step18<false, false, true>(P, P.d, k, s, c[0], c[N], c[2 * N], L.dt, nullptr, &to);
+        pack_to(to, row + (size_t)kCarried * S, S);
+    }
+}
+
+// Reverse sweep of one slot over the segments [seg_lo, seg_hi), last step first, from the tape the
+// recompute launch wrote.  k and kb are the lane's storage for the constants and their adjoints.
+// The adjoint enters from carry (zero for the last group) and leaves to carry, or, for the first
+// group, to ctrl_bar / state_bar / param_bar.
+LFDM_HD void adj_reverse_lane(const AdjLaunch& L, int slot, const PSet& P, K18& k, K18b& kb) {
+    const size_t S = (size_t)L.S, N = (size_t)L.n_user;
+    const int nseg = (L.nsteps + L.K - 1) / L.K;
     load_k(P, L.ovr + slot, S, k);
     const int user = L.slot_user[slot];
     const int uv = user >= 0 ? user : 0;
     S18b lam;
-    s18b_zero(lam);
-    K18b kb;
-    k18b_zero(kb);
-    const int nseg = (L.nsteps + L.K - 1) / L.K;
-    for (int seg = nseg - 1; seg >= 0; --seg) {
+    if (L.seg_hi >= nseg) {
+        s18b_zero(lam);
+        k18b_zero(kb);
+    } else {
+        load_carry(L.carry + slot, S, lam, kb);
+    }
+    for (int seg = L.seg_hi - 1; seg >= L.seg_lo; --seg) {
         const int j0 = seg * L.K;
         const int j1 = (L.nsteps < j0 + L.K ? L.nsteps : j0 + L.K);
-        S18 s;
-        unpack(L.ckpt + (size_t)seg * kCarried * S + slot, S, s);
-        for (int j = j0; j < j1; ++j) {
-            double* row = L.seg + (size_t)(j - j0) * kSegRow * S + slot;
-            pack(s, row, S);
-            const double* c = L.ctrl + (size_t)j * 3 * N + uv;
-            TireOut18 to;
-            step18<false, false, true>(P, P.d, k, s, c[0], c[N], c[2 * N], L.dt, nullptr, &to);
-            pack_to(to, row + (size_t)kCarried * S, S);
-        }
+        const double* blk = L.tape + (size_t)(seg - L.seg_lo) * L.K * kSegRow * S + slot;
         for (int j = j1 - 1; j >= j0; --j) {
-            const double* row = L.seg + (size_t)(j - j0) * kSegRow * S + slot;
+            LFDM_RELOAD_MEMORY();
+            const double* row = blk + (size_t)(j - j0) * kSegRow * S;
+            S18 s;
             unpack(row, S, s);
             TireOut18 to;
             unpack_to(row + (size_t)kCarried * S, S, to);
@@ -145,9 +217,15 @@ LFDM_HD void adj_backward_lane(const AdjLaunch& L, int slot, const PSet& P) {
             }
         }
     }
+    if (L.seg_lo > 0) {
+        store_carry(lam, kb, L.carry + slot, S);
+        return;
+    }
     if (user < 0) return;
-    // adjoint of the initial record (TMeasy layout: 15 fields per tire, vehicle block at 60)
+    // adjoint of the initial record (TMeasy layout: 15 fields per tire, vehicle block at 60); the
+    // fields the dynamics never read are zero
     double* g = L.state_bar + user;
+    for (int f = 0; f < 75; ++f) g[(size_t)f * N] = 0.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         g[(15 * i + 0) * N] = lam.xe[i];

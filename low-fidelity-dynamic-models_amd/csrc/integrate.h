@@ -344,6 +344,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LFDM_WAVES(
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int wave = uniform(slot >> 6);
     if (wave * kWave >= L.S) return;
+    // steps of this lane and of its wavefront in this launch: all of them, or up to the vehicle's
+    // horizon (the wavefront runs until its last lane's horizon, finished lanes idle)
+    int my = L.nsteps, wsteps = L.nsteps;
+    if (L.horizon) {
+        const int h = L.horizon[slot] - L.step_base;
+        my = h < 0 ? 0 : (h > L.nsteps ? L.nsteps : h);
+        const int hw = uniform(L.wave_hmax[wave]) - L.step_base;
+        wsteps = hw < 0 ? 0 : (hw > L.nsteps ? L.nsteps : hw);
+        if (wsteps == 0 && L.n_out == 0) return;
+    }
     const int set = uniform(L.wave_set[wave]);
     const PSet& P = L.sets[set];
     const size_t S = (size_t)L.S;
@@ -356,33 +366,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LFDM_WAVES(
     const int user = L.slot_user[slot];
     const int uv = user >= 0 ? user : 0;  // padding lanes read vehicle 0's controls
     CtrlState<MODE> cs;
-    double t = L.t0;
+    double t = L.t0;  // this vehicle's time
     const double dt = L.dt;
+    // output rows after fewer steps than the vehicle's last are written as the wavefront passes
+    // them; the rest after the last step
     int oi = 0;
     int next_out = L.n_out > 0 ? L.out_steps[0] : -1;
-    auto emit = [&](int done) {
-        if (done != next_out) return;
-        if (user >= 0) M::emit(L.out + (size_t)oi * M::kCh * L.n_user + user, (size_t)L.n_user, t, s);
-        ++oi;
-        next_out = oi < L.n_out ? L.out_steps[oi] : -1;
-    };
-    // all steps but the last keep only the carried state ...
-    const int last = L.nsteps - 1;
-    for (int kk = 0; kk < last; ++kk) {
-        const Ctl c = controls_at<MODE>(L, P, cs, uv, kk, t);
-        M::template step<NR, false>(P, u, k, s, c, dt, nullptr);
-        t = t + dt;
-        emit(kk + 1);
+    // all steps but the lane's last keep only the carried state ...
+    for (int kk = 0; kk < wsteps - 1; ++kk) {
+        if (kk < my - 1) {
+            const Ctl c = controls_at<MODE>(L, P, cs, uv, kk, t);
+            M::template step<NR, false>(P, u, k, s, c, dt, nullptr);
+            t = t + dt;
+        }
+        if (kk + 1 == next_out) {
+            if (kk + 1 < my && user >= 0)
+                M::emit(L.out + (size_t)oi * M::kCh * L.n_user + user, (size_t)L.n_user, t, s);
+            ++oi;
+            next_out = oi < L.n_out ? L.out_steps[oi] : -1;
+        }
     }
-    // ... the last one also returns the right-hand-side quantities the full record holds
-    typename M::X x;
-    {
-        const Ctl c = controls_at<MODE>(L, P, cs, uv, last, t);
+    // ... the last one also returns the right-hand-side quantities the full record holds (a lane
+    // whose horizon comes earlier than its wavefront's takes it at its own step and time)
+    if (my > 0) {
+        typename M::X x;
+        CtrlState<MODE> cl;
+        const Ctl c = controls_at<MODE>(L, P, my == wsteps ? cs : cl, uv, my - 1, t);
         M::template step<NR, true>(P, u, k, s, c, dt, &x);
         t = t + dt;
-        emit(last + 1);
+        M::template store<NR>(L.state, S, slot, P, s, x);
     }
-    M::template store<NR>(L.state, S, slot, P, s, x);
+    // rows at or after the vehicle's last step: its final state, at the row's time
+    if (user >= 0) {
+        double tr = t;
+        int at = my;
+        for (int r = 0; r < L.n_out; ++r) {
+            const int st = L.out_steps[r];
+            if (st < my) continue;
+            for (; at < st; ++at) tr = tr + dt;
+            M::emit(L.out + (size_t)r * M::kCh * L.n_user + user, (size_t)L.n_user, tr, s);
+        }
+    }
 }
 
 template <class M, bool NR>

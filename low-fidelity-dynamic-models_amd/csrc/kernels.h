@@ -29,6 +29,12 @@ struct Launch {
     int ns, np;
     int nsteps, n_out, S, n_user, mode, nr;
     int steps_shared;      // kCtrlSteps: 1 = steps[nsteps][3] shared by every vehicle
+    // per-vehicle horizons (null: every vehicle runs all nsteps): a vehicle stops after
+    // horizon[slot] steps counted from step 0 of the solve; step_base steps were done before this
+    // launch; wave_hmax[wave] is the largest horizon of the wavefront
+    const int* horizon;
+    const int* wave_hmax;
+    int step_base;
 };
 
 // Integration launches of the 11-, 18- and 24-DOF models (one kernel per tire model and control
@@ -39,10 +45,17 @@ hipError_t launch_d24(const Launch& L, int block, hipStream_t stream);
 
 // Differentiation form (18-DOF TMeasy).  Forward: from the record state, nsteps steps with per-step
 // controls ctrl[nsteps][3][n_user]; writes xy[nsteps][2][n_user] (position after each step) and
-// the carried state every K steps into ckpt[ceil(nsteps/K)][32][S].  Backward: from xy_bar (same
-// layout as xy) writes ctrl_bar (layout of ctrl), state_bar[75][n_user] (adjoint of the initial
-// record) and param_bar[7][n_user] (m, jz, jx, dfy0Pn, dfy0P2n, torque scale, maxBrakeTorque);
-// seg is a [K][56][S] work buffer (carried state and tire outputs of every step of a segment).
+// the carried state every K steps into ckpt[ceil(nsteps/K)][32][S].  Backward, one group of
+// segments [seg_lo, seg_hi) at a time, last group first: the recompute launch reruns the group's
+// segments from their checkpoints into tape[seg_hi - seg_lo][K][56][S] (carried state and tire
+// outputs of every step), the reverse launch walks them backwards.  From xy_bar (layout of xy) the
+// last group's reverse writes ctrl_bar (layout of ctrl), state_bar[75][n_user] (adjoint of the
+// initial record) and param_bar[7][n_user] (m, jz, jx, dfy0Pn, dfy0P2n, torque scale,
+// maxBrakeTorque); between groups the running adjoints live in carry[52][S].
+constexpr int kAdjCkptRow = 32;  // doubles per vehicle in a checkpoint
+constexpr int kAdjTapeRow = 56;  // doubles per vehicle and step in the tape
+constexpr int kAdjCarry = 52;    // doubles per vehicle in carry
+
 struct AdjLaunch {
     const PSet* sets;
     const int* wave_set;
@@ -56,13 +69,18 @@ struct AdjLaunch {
     double* state_bar;
     double* param_bar;
     double* ckpt;
-    double* seg;
+    double* tape;
+    double* carry;
     double dt;
     int nsteps, K, S, n_user;
+    int seg_lo, seg_hi;
 };
 
-hipError_t launch_adj_forward(const AdjLaunch& L, hipStream_t stream);
-hipError_t launch_adj_backward(const AdjLaunch& L, hipStream_t stream);
+// waves_per_simd 1 or 2: register budget of the forward kernel (1 when the fleet fills at most one
+// wavefront per SIMD, so the step runs without spills).
+hipError_t launch_adj_forward(const AdjLaunch& L, int waves_per_simd, hipStream_t stream);
+hipError_t launch_adj_recompute(const AdjLaunch& L, hipStream_t stream);
+hipError_t launch_adj_reverse(const AdjLaunch& L, hipStream_t stream);
 
 // Gathers full records of `n` slots from SoA state into out[n][rec].
 hipError_t launch_gather(const double* state, int S, int rec, const int* slots, int n, double* out,

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <numeric>
 #include <random>
 #include <sstream>
@@ -106,6 +107,10 @@ struct lfdm_solver {
     std::vector<double> ovr_host;  // [7][total]
     bool ovr_dirty = false;
     double t_now = 0.0;
+    // per-vehicle horizons (steps from t = 0; empty: none) and the slot layout
+    std::vector<int> horizon_host;
+    bool horizon_dirty = false;
+    int layout = LFDM_LAYOUT_GROUPED;
     // device
     bool built = false;
     int S = 0, rec = 0;
@@ -118,6 +123,8 @@ struct lfdm_solver {
     int* d_user_slot = nullptr;
     double* d_state = nullptr;
     double* d_ovr = nullptr;
+    int* d_horizon = nullptr;    // [S]
+    int* d_wave_hmax = nullptr;  // [S / 64]
     DrvRow* d_drv = nullptr;
     double* d_knots = nullptr;
     double* d_steps = nullptr;
@@ -132,13 +139,37 @@ struct lfdm_solver {
     // differentiation form
     double* d_ckpt = nullptr;
     size_t ckpt_cap = 0;
-    double* d_seg = nullptr;
-    size_t seg_cap = 0;
+    double* d_tape = nullptr;
+    size_t tape_cap = 0;
+    double* d_carry = nullptr;
+    size_t carry_cap = 0;
     int adj_nsteps = 0, adj_K = 0;
+    size_t adj_tape_budget = size_t(16) << 30;  // bytes the backward's tape may take
+    int simds = 0;  // SIMDs of the device (4 per CU), queried once
     const double* adj_ctrl = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     double last_ms = 0.0;
+};
+
+// A mixed fleet: solver handles (buckets, typically one per model) integrated together by one HIP
+// graph, one branch per bucket.
+struct lfdm_fleet {
+    struct Cfg {
+        int block = 256;
+        int spl = 0;  // steps per launch, 0 = the whole run in one launch
+    };
+    std::vector<lfdm_solver*> buckets;
+    std::vector<Cfg> cfg;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t> pre, join;
+    hipEvent_t fork = nullptr, post = nullptr, ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<char> sig;  // launch records and block sizes the graph was captured with
+    bool timed = false;
 };
 
 namespace {
@@ -196,11 +227,17 @@ int ensure_device(lfdm_solver* h) {
     h->user_slot.assign(h->total, 0);
     std::vector<PSet> sets(h->groups.size());
     std::vector<DrvRow> pool;
+    const bool by_horizon = h->layout == LFDM_LAYOUT_BY_HORIZON && !h->horizon_host.empty();
     for (size_t g = 0; g < h->groups.size(); ++g) {
         const Group& G = h->groups[g];
+        std::vector<int> order(G.n);
+        std::iota(order.begin(), order.end(), (int)G.user_base);
+        if (by_horizon)  // longest horizon first: lanes of a wavefront stop together, long wavefronts start first
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int a, int b) { return h->horizon_host[a] > h->horizon_host[b]; });
         for (unsigned i = 0; i < G.n; ++i) {
-            slot_user[slot_base[g] + i] = (int)(G.user_base + i);
-            h->user_slot[G.user_base + i] = slot_base[g] + (int)i;
+            slot_user[slot_base[g] + i] = order[i];
+            h->user_slot[order[i]] = slot_base[g] + (int)i;
         }
         const int waves = (int)((G.n + kWave - 1) / kWave);
         for (int w = 0; w < waves; ++w) wave_set[slot_base[g] / kWave + w] = (int)g;
@@ -250,6 +287,29 @@ int ensure_device(lfdm_solver* h) {
     HIPCHK(hipStreamSynchronize(h->stream));
     h->built = true;
     h->ovr_dirty = false;
+    h->horizon_dirty = true;
+    return LFDM_OK;
+}
+
+// Drops the device image (keeping the current states as the records to rebuild it from), so the
+// next call lays the slots out again.
+int drop_device_image(lfdm_solver* h) {
+    if (!h->built) return LFDM_OK;
+    std::vector<double> cur((size_t)h->total * h->rec);
+    if (int rc = lfdm_get_states(h, cur.data())) return rc;
+    h->init.swap(cur);
+    h->n_states = h->total;
+    dfree(h->d_sets);
+    dfree(h->d_wave_set);
+    dfree(h->d_slot_user);
+    dfree(h->d_user_slot);
+    dfree(h->d_state);
+    dfree(h->d_ovr);
+    dfree(h->d_drv);
+    dfree(h->d_horizon);
+    dfree(h->d_wave_hmax);
+    h->built = false;
+    h->adj_nsteps = 0;  // a pending adjoint refers to the old slots
     return LFDM_OK;
 }
 
@@ -267,6 +327,25 @@ int sync_inputs(lfdm_solver* h) {
         HIPCHK(hipMemcpyAsync(h->d_ovr, ovr.data(), ovr.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
         h->ovr_dirty = false;
+    }
+    if (h->horizon_dirty) {
+        dfree(h->d_horizon);
+        dfree(h->d_wave_hmax);
+        if (!h->horizon_host.empty()) {
+            std::vector<int> hs(h->S, 0), wm(h->S / kWave, 0);
+            for (unsigned u = 0; u < h->total; ++u) {
+                const int slot = h->user_slot[u];
+                hs[slot] = h->horizon_host[u];
+                wm[slot / kWave] = std::max(wm[slot / kWave], hs[slot]);
+            }
+            HIPCHK(hipMalloc((void**)&h->d_horizon, hs.size() * sizeof(int)));
+            HIPCHK(hipMalloc((void**)&h->d_wave_hmax, wm.size() * sizeof(int)));
+            HIPCHK(hipMemcpyAsync(h->d_horizon, hs.data(), hs.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+            HIPCHK(hipMemcpyAsync(h->d_wave_hmax, wm.data(), wm.size() * sizeof(int), hipMemcpyHostToDevice,
+                                  h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+        }
+        h->horizon_dirty = false;
     }
     if (h->knots_dirty) {
         dfree(h->d_knots);
@@ -316,9 +395,12 @@ std::vector<int> output_steps(int nsteps, double step, double dtout) {
     return r;
 }
 
-int launch(lfdm_solver* h, int mode, double t0, int nsteps, const std::vector<int>* outs, double cs = 0,
-           double ct = 0, double cb = 0, const double* steps_ptr = nullptr) {
-    Launch L{};
+// Launch record of one integration launch of h (no outputs).  Zero-filled first, so records can be
+// compared bytewise.
+Launch make_launch(lfdm_solver* h, int mode, double t0, int nsteps, double cs = 0, double ct = 0, double cb = 0,
+                   const double* steps_ptr = nullptr) {
+    Launch L;
+    std::memset(&L, 0, sizeof(L));
     L.sets = h->d_sets;
     L.wave_set = h->d_wave_set;
     L.slot_user = h->d_slot_user;
@@ -342,6 +424,26 @@ int launch(lfdm_solver* h, int mode, double t0, int nsteps, const std::vector<in
     L.mode = mode;
     L.nr = h->tire_type == LFDM_TIRE_TMEASYNR;
     L.steps_shared = (mode == kCtrlSteps && h->steps_shared) ? 1 : 0;
+    if (h->d_horizon) {
+        L.horizon = h->d_horizon;
+        L.wave_hmax = h->d_wave_hmax;
+        L.step_base = (int)std::llround(t0 / h->step);
+    }
+    return L;
+}
+
+int block_size(unsigned tpb) {
+    const int block = (int)((tpb + kWave - 1) / kWave) * kWave;
+    return std::min(256, std::max(64, block));
+}
+
+hipError_t launch_dof(int dof, const Launch& L, int block, hipStream_t stream) {
+    return dof == 11 ? launch_d11(L, block, stream) : dof == 18 ? launch_d18(L, block, stream) : launch_d24(L, block, stream);
+}
+
+int launch(lfdm_solver* h, int mode, double t0, int nsteps, const std::vector<int>* outs, double cs = 0,
+           double ct = 0, double cb = 0, const double* steps_ptr = nullptr) {
+    Launch L = make_launch(h, mode, t0, nsteps, cs, ct, cb, steps_ptr);
     if (outs && !outs->empty()) {
         h->outs_host = *outs;
         outs = &h->outs_host;
@@ -356,12 +458,8 @@ int launch(lfdm_solver* h, int mode, double t0, int nsteps, const std::vector<in
         L.n_out = (int)outs->size();
         L.out = h->d_out;
     }
-    int block = (int)((h->tpb + kWave - 1) / kWave) * kWave;
-    block = std::min(256, std::max(64, block));
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    hipError_t e = h->dof == 11 ? launch_d11(L, block, h->stream)
-                   : h->dof == 18 ? launch_d18(L, block, h->stream)
-                                  : launch_d24(L, block, h->stream);
+    hipError_t e = launch_dof(h->dof, L, block_size(h->tpb), h->stream);
     if (e != hipSuccess) return fail(LFDM_E_HIP, std::string("integration launch failed: ") + hipGetErrorString(e));
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     h->timed = true;
@@ -410,6 +508,55 @@ int csv_flush(lfdm_solver* h) {
         h->csv[i].clear();
     }
     return LFDM_OK;
+}
+
+// Control source of h ready for nsteps steps from step 0 (what Solve checks).
+int check_controls(lfdm_solver* h, int nsteps) {
+    if (h->ctrl_mode == kCtrlTable)
+        for (const Group& g : h->groups)
+            if (g.rows.empty())
+                return fail(LFDM_E_STATE, "No controls provided, please use construct to pass path to driver inputs");
+    if (h->ctrl_mode == kCtrlSteps && nsteps > h->steps_n)
+        return fail(LFDM_E_STATE, "per-step controls cover " + std::to_string(h->steps_n) + " steps, " +
+                                      std::to_string(nsteps) + " requested");
+    return LFDM_OK;
+}
+
+// Launch records that integrate h nsteps steps from t = 0 (step 0) in launches of at most spl
+// steps, the time accumulated step by step as Solve does.  Ends with the final time in *t_end.
+std::vector<Launch> run_launches(lfdm_solver* h, int nsteps, int spl, double* t_end) {
+    const double* steps = h->ctrl_mode == kCtrlSteps ? (h->steps_ext ? h->steps_ext : h->d_steps) : nullptr;
+    const size_t row = h->steps_shared ? 3 : 3 * (size_t)h->total;
+    std::vector<Launch> r;
+    double t = 0.0;
+    for (int done = 0; done < nsteps;) {
+        const int n = std::min(spl > 0 ? spl : nsteps, nsteps - done);
+        Launch L = make_launch(h, h->ctrl_mode, t, n, 0, 0, 0, steps ? steps + (size_t)done * row : nullptr);
+        if (L.horizon) L.step_base = done;
+        r.push_back(L);
+        for (int k = 0; k < n; ++k) t += h->step;
+        done += n;
+    }
+    if (t_end) *t_end = t;
+    return r;
+}
+
+// Lane-steps a layout would run: sum over wavefronts of 64 x (the wavefront's longest horizon,
+// capped at nsteps); vehicles without horizons run nsteps.
+double predicted_lane_steps(const lfdm_solver* h, int layout, int nsteps) {
+    double sum = 0;
+    for (const Group& G : h->groups) {
+        std::vector<int> hz(G.n, nsteps);
+        if (!h->horizon_host.empty())
+            for (unsigned i = 0; i < G.n; ++i) hz[i] = std::min(h->horizon_host[G.user_base + i], nsteps);
+        if (layout == LFDM_LAYOUT_BY_HORIZON) std::stable_sort(hz.begin(), hz.end(), std::greater<int>());
+        for (size_t w = 0; w < hz.size(); w += kWave) {
+            int m = 0;
+            for (size_t i = w; i < std::min(hz.size(), w + kWave); ++i) m = std::max(m, hz[i]);
+            sum += (double)kWave * m;
+        }
+    }
+    return sum;
 }
 
 }  // namespace
@@ -480,8 +627,11 @@ int lfdm_destroy(lfdm_solver* h) {
     dfree(h->d_out_steps);
     dfree(h->d_out);
     dfree(h->d_rec);
+    dfree(h->d_horizon);
+    dfree(h->d_wave_hmax);
     dfree(h->d_ckpt);
-    dfree(h->d_seg);
+    dfree(h->d_tape);
+    dfree(h->d_carry);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
@@ -888,6 +1038,39 @@ int lfdm_set_time(lfdm_solver* h, double t) {
     return LFDM_OK;
 }
 
+int lfdm_set_horizons(lfdm_solver* h, const int* steps) {
+    if (int rc = check(h)) return rc;
+    std::vector<int> hz;
+    if (steps) {
+        hz.assign(steps, steps + h->total);
+        for (int v : hz)
+            if (v < 0) return fail(LFDM_E_ARG, "horizons must be non-negative step counts");
+    }
+    const bool relayout = h->layout == LFDM_LAYOUT_BY_HORIZON && (!hz.empty() || !h->horizon_host.empty());
+    if (relayout)
+        if (int rc = drop_device_image(h)) return rc;
+    h->horizon_host.swap(hz);
+    h->horizon_dirty = true;
+    return LFDM_OK;
+}
+
+int lfdm_set_layout(lfdm_solver* h, int layout) {
+    if (int rc = check(h)) return rc;
+    if (layout != LFDM_LAYOUT_GROUPED && layout != LFDM_LAYOUT_BY_HORIZON) return fail(LFDM_E_ARG, "bad layout");
+    if (layout == h->layout) return LFDM_OK;
+    if (!h->horizon_host.empty())
+        if (int rc = drop_device_image(h)) return rc;
+    h->layout = layout;
+    return LFDM_OK;
+}
+
+int lfdm_get_layout(lfdm_solver* h, int* layout) {
+    if (int rc = check(h)) return rc;
+    if (!layout) return fail(LFDM_E_ARG, "null output");
+    *layout = h->layout;
+    return LFDM_OK;
+}
+
 int lfdm_advance(lfdm_solver* h, int nsteps) {
     if (int rc = check(h)) return rc;
     if (nsteps <= 0) return fail(LFDM_E_ARG, "step count must be positive");
@@ -930,13 +1113,36 @@ AdjLaunch adj_launch(lfdm_solver* h) {
     L.ovr = h->d_ovr;
     L.ctrl = h->adj_ctrl;
     L.ckpt = h->d_ckpt;
-    L.seg = h->d_seg;
+    L.tape = h->d_tape;
+    L.carry = h->d_carry;
     L.dt = h->step;
     L.nsteps = h->adj_nsteps;
     L.K = h->adj_K;
     L.S = h->S;
     L.n_user = (int)h->total;
     return L;
+}
+
+int simd_count(lfdm_solver* h) {
+    if (h->simds > 0) return h->simds;
+    int dev = 0, cus = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    h->simds = 4 * std::max(cus, 1);
+    return h->simds;
+}
+
+// Segments per backward group: enough (segment, vehicle) lanes for about four wavefronts per SIMD
+// in the recompute launch, within the handle's tape budget (at least one segment).
+int adj_group(lfdm_solver* h, int nseg) {
+    const int simds = simd_count(h);
+    if (simds < 0) return simds;
+    const size_t waves = (size_t)h->S / kWave;
+    const size_t want = 4 * (size_t)simds;
+    size_t G = (want + waves - 1) / std::max<size_t>(waves, 1);
+    const size_t per_seg = (size_t)h->adj_K * kAdjTapeRow * (size_t)h->S * sizeof(double);
+    G = std::min(G, std::max<size_t>(h->adj_tape_budget / std::max<size_t>(per_seg, 1), 1));
+    return (int)std::max<size_t>(1, std::min<size_t>(G, (size_t)nseg));
 }
 
 }  // namespace
@@ -949,18 +1155,27 @@ int lfdm_adjoint_forward(lfdm_solver* h, int nsteps, int checkpoint_every, const
     if (int rc = sync_inputs(h)) return rc;
     const int K = std::min(checkpoint_every, nsteps);
     const size_t nseg = (size_t)((nsteps + K - 1) / K);
-    if (int rc = ensure_buffer(h->d_ckpt, h->ckpt_cap, nseg * 32 * (size_t)h->S)) return rc;
-    if (int rc = ensure_buffer(h->d_seg, h->seg_cap, (size_t)K * 56 * (size_t)h->S)) return rc;
+    if (int rc = ensure_buffer(h->d_ckpt, h->ckpt_cap, nseg * kAdjCkptRow * (size_t)h->S)) return rc;
+    const int simds = simd_count(h);
+    if (simds < 0) return simds;
     h->adj_nsteps = nsteps;
     h->adj_K = K;
     h->adj_ctrl = dctrl;
     AdjLaunch L = adj_launch(h);
     L.xy = dxy;
+    // a fleet of at most one wavefront per SIMD gets the whole register file per lane
+    const int waves = (size_t)h->S / kWave <= (size_t)simds ? 1 : 2;
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    hipError_t e = launch_adj_forward(L, h->stream);
+    hipError_t e = launch_adj_forward(L, waves, h->stream);
     if (e != hipSuccess) return fail(LFDM_E_HIP, std::string("adjoint forward launch failed: ") + hipGetErrorString(e));
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     h->timed = true;
+    return LFDM_OK;
+}
+
+int lfdm_adjoint_set_tape_budget(lfdm_solver* h, unsigned long long bytes) {
+    if (int rc = check(h)) return rc;
+    h->adj_tape_budget = (size_t)bytes;
     return LFDM_OK;
 }
 
@@ -969,16 +1184,25 @@ int lfdm_adjoint_backward(lfdm_solver* h, const double* dxy_bar, double* dctrl_b
     if (int rc = check(h)) return rc;
     if (!dxy_bar || !dctrl_bar || !dstate_bar || !dparam_bar) return fail(LFDM_E_ARG, "null adjoint buffer");
     if (h->adj_nsteps <= 0) return fail(LFDM_E_STATE, "adjoint backward before adjoint forward");
+    const int nseg = (h->adj_nsteps + h->adj_K - 1) / h->adj_K;
+    const int G = adj_group(h, nseg);
+    if (G < 0) return G;
+    if (int rc = ensure_buffer(h->d_tape, h->tape_cap, (size_t)G * h->adj_K * kAdjTapeRow * (size_t)h->S)) return rc;
+    if (int rc = ensure_buffer(h->d_carry, h->carry_cap, (size_t)kAdjCarry * (size_t)h->S)) return rc;
     AdjLaunch L = adj_launch(h);
     L.xy_bar = dxy_bar;
     L.ctrl_bar = dctrl_bar;
     L.state_bar = dstate_bar;
     L.param_bar = dparam_bar;
-    // fields of the initial-record adjoint that the dynamics never read stay zero
-    HIPCHK(hipMemsetAsync(dstate_bar, 0, (size_t)h->rec * h->total * sizeof(double), h->stream));
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    hipError_t e = launch_adj_backward(L, h->stream);
-    if (e != hipSuccess) return fail(LFDM_E_HIP, std::string("adjoint backward launch failed: ") + hipGetErrorString(e));
+    for (int hi = nseg; hi > 0; hi -= G) {
+        L.seg_hi = hi;
+        L.seg_lo = std::max(hi - G, 0);
+        hipError_t e = launch_adj_recompute(L, h->stream);
+        if (e == hipSuccess) e = launch_adj_reverse(L, h->stream);
+        if (e != hipSuccess)
+            return fail(LFDM_E_HIP, std::string("adjoint backward launch failed: ") + hipGetErrorString(e));
+    }
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     h->timed = true;
     return LFDM_OK;
@@ -1012,6 +1236,266 @@ int lfdm_last_launch_ms(lfdm_solver* h, double* ms) {
     float f = 0;
     HIPCHK(hipEventElapsedTime(&f, h->ev0, h->ev1));
     *ms = f;
+    return LFDM_OK;
+}
+
+// ---- mixed fleets ------------------------------------------------------------------------------
+
+namespace {
+
+int fleet_check(lfdm_fleet* f) {
+    if (!f) return fail(LFDM_E_ARG, "null fleet handle");
+    return LFDM_OK;
+}
+
+void fleet_drop_graph(lfdm_fleet* f) {
+    if (f->exec) (void)hipGraphExecDestroy(f->exec);
+    if (f->graph) (void)hipGraphDestroy(f->graph);
+    f->exec = nullptr;
+    f->graph = nullptr;
+    f->sig.clear();
+}
+
+int fleet_prepare(lfdm_fleet* f, int nsteps) {
+    if (f->buckets.empty()) return fail(LFDM_E_STATE, "the fleet has no buckets");
+    if (nsteps <= 0) return fail(LFDM_E_ARG, "step count must be positive");
+    for (lfdm_solver* h : f->buckets) {
+        if (int rc = ensure_device(h)) return rc;
+        if (int rc = sync_inputs(h)) return rc;
+        if (int rc = check_controls(h, nsteps)) return rc;
+    }
+    if (!f->stream) {
+        HIPCHK(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+        f->own_stream = true;
+    }
+    if (!f->fork) {
+        HIPCHK(hipEventCreateWithFlags(&f->fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&f->post, hipEventDisableTiming));
+        HIPCHK(hipEventCreate(&f->ev0));
+        HIPCHK(hipEventCreate(&f->ev1));
+    }
+    while (f->branch.size() < f->buckets.size()) {
+        hipStream_t st;
+        hipEvent_t a, b;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+        f->branch.push_back(st);
+        f->pre.push_back(a);
+        f->join.push_back(b);
+    }
+    return LFDM_OK;
+}
+
+// The fleet's stream waits for work pending on the buckets' own streams (and, after fleet_end,
+// the buckets' streams wait for the fleet's).
+int fleet_begin(lfdm_fleet* f) {
+    for (size_t b = 0; b < f->buckets.size(); ++b) {
+        HIPCHK(hipEventRecord(f->pre[b], f->buckets[b]->stream));
+        HIPCHK(hipStreamWaitEvent(f->stream, f->pre[b], 0));
+    }
+    return LFDM_OK;
+}
+
+int fleet_end(lfdm_fleet* f) {
+    HIPCHK(hipEventRecord(f->post, f->stream));
+    for (lfdm_solver* h : f->buckets) HIPCHK(hipStreamWaitEvent(h->stream, f->post, 0));
+    return LFDM_OK;
+}
+
+int launch_list(lfdm_solver* h, const std::vector<Launch>& ls, int block, hipStream_t st) {
+    for (const Launch& L : ls) {
+        hipError_t e = launch_dof(h->dof, L, block, st);
+        if (e != hipSuccess) return fail(LFDM_E_HIP, std::string("integration launch failed: ") + hipGetErrorString(e));
+    }
+    return LFDM_OK;
+}
+
+}  // namespace
+
+int lfdm_fleet_create(lfdm_fleet** out) {
+    if (!out) return fail(LFDM_E_ARG, "null output handle");
+    *out = new lfdm_fleet();
+    return LFDM_OK;
+}
+
+int lfdm_fleet_destroy(lfdm_fleet* f) {
+    if (!f) return LFDM_OK;
+    if (f->stream) (void)hipStreamSynchronize(f->stream);
+    fleet_drop_graph(f);
+    for (hipStream_t st : f->branch) (void)hipStreamDestroy(st);
+    for (hipEvent_t e : f->pre) (void)hipEventDestroy(e);
+    for (hipEvent_t e : f->join) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {f->fork, f->post, f->ev0, f->ev1})
+        if (e) (void)hipEventDestroy(e);
+    if (f->own_stream && f->stream) (void)hipStreamDestroy(f->stream);
+    delete f;
+    return LFDM_OK;
+}
+
+int lfdm_fleet_add(lfdm_fleet* f, lfdm_solver* h) {
+    if (int rc = fleet_check(f)) return rc;
+    if (int rc = check(h)) return rc;
+    for (lfdm_solver* b : f->buckets)
+        if (b == h) return fail(LFDM_E_ARG, "solver already in the fleet");
+    f->buckets.push_back(h);
+    f->cfg.emplace_back();
+    fleet_drop_graph(f);
+    return (int)f->buckets.size() - 1;
+}
+
+int lfdm_fleet_set_stream(lfdm_fleet* f, void* stream) {
+    if (int rc = fleet_check(f)) return rc;
+    if (f->own_stream && f->stream) {
+        (void)hipStreamSynchronize(f->stream);
+        (void)hipStreamDestroy(f->stream);
+    }
+    f->stream = (hipStream_t)stream;
+    f->own_stream = false;
+    return LFDM_OK;
+}
+
+int lfdm_fleet_set_config(lfdm_fleet* f, int bucket, int block, int steps_per_launch) {
+    if (int rc = fleet_check(f)) return rc;
+    if (bucket < 0 || bucket >= (int)f->buckets.size()) return fail(LFDM_E_ARG, "bad bucket index");
+    if (block < 64 || block > 256 || block % kWave) return fail(LFDM_E_ARG, "block size must be 64, 128, 192 or 256");
+    if (steps_per_launch < 0) return fail(LFDM_E_ARG, "steps per launch must be >= 0");
+    f->cfg[bucket].block = block;
+    f->cfg[bucket].spl = steps_per_launch;
+    return LFDM_OK;
+}
+
+int lfdm_fleet_get_config(lfdm_fleet* f, int bucket, int* block, int* steps_per_launch, int* layout) {
+    if (int rc = fleet_check(f)) return rc;
+    if (bucket < 0 || bucket >= (int)f->buckets.size()) return fail(LFDM_E_ARG, "bad bucket index");
+    if (block) *block = f->cfg[bucket].block;
+    if (steps_per_launch) *steps_per_launch = f->cfg[bucket].spl;
+    if (layout) *layout = f->buckets[bucket]->layout;
+    return LFDM_OK;
+}
+
+int lfdm_fleet_run(lfdm_fleet* f, int nsteps) {
+    if (int rc = fleet_check(f)) return rc;
+    if (int rc = fleet_prepare(f, nsteps)) return rc;
+    std::vector<std::vector<Launch>> ls(f->buckets.size());
+    std::vector<double> t_end(f->buckets.size());
+    std::vector<char> sig;
+    for (size_t b = 0; b < f->buckets.size(); ++b) {
+        ls[b] = run_launches(f->buckets[b], nsteps, f->cfg[b].spl, &t_end[b]);
+        const char* blk = reinterpret_cast<const char*>(&f->cfg[b].block);
+        sig.insert(sig.end(), blk, blk + sizeof(int));
+        for (const Launch& L : ls[b]) {
+            const char* p = reinterpret_cast<const char*>(&L);
+            sig.insert(sig.end(), p, p + sizeof(Launch));
+        }
+    }
+    if (!f->exec || sig != f->sig) {
+        fleet_drop_graph(f);
+        // one branch per bucket: fork from the fleet's stream, a chain of launches, join
+        HIPCHK(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
+        int rc = LFDM_OK;
+        hipError_t e = hipEventRecord(f->fork, f->stream);
+        for (size_t b = 0; e == hipSuccess && rc == LFDM_OK && b < f->buckets.size(); ++b) {
+            e = hipStreamWaitEvent(f->branch[b], f->fork, 0);
+            if (e == hipSuccess) rc = launch_list(f->buckets[b], ls[b], f->cfg[b].block, f->branch[b]);
+            if (e == hipSuccess && rc == LFDM_OK) e = hipEventRecord(f->join[b], f->branch[b]);
+            if (e == hipSuccess && rc == LFDM_OK) e = hipStreamWaitEvent(f->stream, f->join[b], 0);
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(f->stream, &g);
+        if (rc != LFDM_OK || e != hipSuccess || ee != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (rc != LFDM_OK) return rc;
+            return fail(LFDM_E_HIP, std::string("graph capture failed: ") +
+                                        hipGetErrorString(e != hipSuccess ? e : ee));
+        }
+        f->graph = g;
+        HIPCHK(hipGraphInstantiate(&f->exec, f->graph, nullptr, nullptr, 0));
+        f->sig.swap(sig);
+    }
+    if (int rc = fleet_begin(f)) return rc;
+    HIPCHK(hipEventRecord(f->ev0, f->stream));
+    HIPCHK(hipGraphLaunch(f->exec, f->stream));
+    HIPCHK(hipEventRecord(f->ev1, f->stream));
+    f->timed = true;
+    if (int rc = fleet_end(f)) return rc;
+    for (size_t b = 0; b < f->buckets.size(); ++b) f->buckets[b]->t_now = t_end[b];
+    return LFDM_OK;
+}
+
+int lfdm_fleet_autotune(lfdm_fleet* f, int nsteps, int trial_steps) {
+    if (int rc = fleet_check(f)) return rc;
+    if (trial_steps <= 0) return fail(LFDM_E_ARG, "trial step count must be positive");
+    if (f->buckets.empty()) return fail(LFDM_E_STATE, "the fleet has no buckets");
+    for (size_t b = 0; b < f->buckets.size(); ++b) {
+        lfdm_solver* h = f->buckets[b];
+        // layout from the lane-steps each would run (results do not depend on it)
+        if (!h->horizon_host.empty()) {
+            const double g = predicted_lane_steps(h, LFDM_LAYOUT_GROUPED, nsteps);
+            const double s = predicted_lane_steps(h, LFDM_LAYOUT_BY_HORIZON, nsteps);
+            if (int rc = lfdm_set_layout(h, s < 0.99 * g ? LFDM_LAYOUT_BY_HORIZON : LFDM_LAYOUT_GROUPED)) return rc;
+        }
+    }
+    const int T = std::min(trial_steps, nsteps);
+    if (int rc = fleet_prepare(f, nsteps)) return rc;
+    if (int rc = fleet_begin(f)) return rc;
+    // launch shape from timed trials of T steps from the current states (restored after each)
+    for (size_t b = 0; b < f->buckets.size(); ++b) {
+        lfdm_solver* h = f->buckets[b];
+        const size_t n = (size_t)h->rec * h->S;
+        double* saved = nullptr;
+        HIPCHK(hipMalloc((void**)&saved, n * sizeof(double)));
+        hipError_t e = hipMemcpyAsync(saved, h->d_state, n * sizeof(double), hipMemcpyDeviceToDevice, f->stream);
+        float best = 0;
+        lfdm_fleet::Cfg pick = f->cfg[b];
+        int rc = LFDM_OK;
+        const int blocks[3] = {64, 128, 256};
+        const int parts[2] = {1, 4};
+        for (int bi = 0; bi < 3 && rc == LFDM_OK && e == hipSuccess; ++bi)
+            for (int pi = 0; pi < 2 && rc == LFDM_OK && e == hipSuccess; ++pi) {
+                const std::vector<Launch> ls = run_launches(h, T, (T + parts[pi] - 1) / parts[pi], nullptr);
+                float ms = 0;
+                for (int rep = 0; rep < 2 && rc == LFDM_OK && e == hipSuccess; ++rep) {
+                    e = hipMemcpyAsync(h->d_state, saved, n * sizeof(double), hipMemcpyDeviceToDevice, f->stream);
+                    if (e == hipSuccess) e = hipEventRecord(f->ev0, f->stream);
+                    if (e == hipSuccess) rc = launch_list(h, ls, blocks[bi], f->stream);
+                    if (e == hipSuccess && rc == LFDM_OK) e = hipEventRecord(f->ev1, f->stream);
+                    if (e == hipSuccess && rc == LFDM_OK) e = hipEventSynchronize(f->ev1);
+                    float x = 0;
+                    if (e == hipSuccess && rc == LFDM_OK) e = hipEventElapsedTime(&x, f->ev0, f->ev1);
+                    if (rep == 0 || x < ms) ms = x;
+                }
+                if (e == hipSuccess && rc == LFDM_OK && (best == 0 || ms < best)) {
+                    best = ms;
+                    pick.block = blocks[bi];
+                    pick.spl = parts[pi] == 1 ? 0 : (nsteps + parts[pi] - 1) / parts[pi];
+                }
+            }
+        if (e == hipSuccess) e = hipMemcpyAsync(h->d_state, saved, n * sizeof(double), hipMemcpyDeviceToDevice, f->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
+        (void)hipFree(saved);
+        if (rc != LFDM_OK) return rc;
+        if (e != hipSuccess) return fail(LFDM_E_HIP, std::string("autotune trial failed: ") + hipGetErrorString(e));
+        f->cfg[b] = pick;
+    }
+    f->timed = false;
+    return fleet_end(f);
+}
+
+int lfdm_fleet_synchronize(lfdm_fleet* f) {
+    if (int rc = fleet_check(f)) return rc;
+    if (f->stream) HIPCHK(hipStreamSynchronize(f->stream));
+    return LFDM_OK;
+}
+
+int lfdm_fleet_last_ms(lfdm_fleet* f, double* ms) {
+    if (int rc = fleet_check(f)) return rc;
+    if (!ms) return fail(LFDM_E_ARG, "null output");
+    if (!f->timed) return fail(LFDM_E_STATE, "no fleet run recorded");
+    HIPCHK(hipEventSynchronize(f->ev1));
+    float x = 0;
+    HIPCHK(hipEventElapsedTime(&x, f->ev0, f->ev1));
+    *ms = x;
     return LFDM_OK;
 }
 

@@ -12,6 +12,8 @@ LIB_PATH = os.environ.get("LFDM_AMD_LIB") or os.path.join(_HERE, "liblfdm_amd.so
 LFDM_OK = 0
 LFDM_TIRE_TMEASY = 0
 LFDM_TIRE_TMEASYNR = 1
+LFDM_LAYOUT_GROUPED = 0
+LFDM_LAYOUT_BY_HORIZON = 1
 ERRORS = {
     -1: "LFDM_E_ARG",
     -2: "LFDM_E_IO",
@@ -94,9 +96,23 @@ SIGNATURES = {
     "lfdm_set_time": [_p, _d],
     "lfdm_adjoint_forward": [_p, _i, _i, _p, _p],
     "lfdm_adjoint_backward": [_p, _p, _p, _p, _p],
+    "lfdm_adjoint_set_tape_budget": [_p, ctypes.c_ulonglong],
     "lfdm_set_stream": [_p, _p],
     "lfdm_synchronize": [_p],
     "lfdm_last_launch_ms": [_p, _pd],
+    "lfdm_set_horizons": [_p, _p],
+    "lfdm_set_layout": [_p, _i],
+    "lfdm_get_layout": [_p, ctypes.POINTER(_i)],
+    "lfdm_fleet_create": [ctypes.POINTER(_p)],
+    "lfdm_fleet_destroy": [_p],
+    "lfdm_fleet_add": [_p, _p],
+    "lfdm_fleet_set_stream": [_p, _p],
+    "lfdm_fleet_set_config": [_p, _i, _i, _i],
+    "lfdm_fleet_get_config": [_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],
+    "lfdm_fleet_autotune": [_p, _i, _i],
+    "lfdm_fleet_run": [_p, _i],
+    "lfdm_fleet_synchronize": [_p],
+    "lfdm_fleet_last_ms": [_p, _pd],
 }
 
 for _name, _args in SIGNATURES.items():

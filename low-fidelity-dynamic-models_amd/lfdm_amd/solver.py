@@ -11,10 +11,12 @@ import ctypes
 
 import numpy as np
 
-from ._lib import LFDM_TIRE_TMEASY, LFDM_TIRE_TMEASYNR, check, enc, lib
+from ._lib import LFDM_LAYOUT_BY_HORIZON, LFDM_LAYOUT_GROUPED, LFDM_TIRE_TMEASY, LFDM_TIRE_TMEASYNR, check, enc, lib
 
 TMeasy = LFDM_TIRE_TMEASY
 TMeasyNr = LFDM_TIRE_TMEASYNR
+LAYOUT_GROUPED = LFDM_LAYOUT_GROUPED
+LAYOUT_BY_HORIZON = LFDM_LAYOUT_BY_HORIZON
 
 # Field names of the records (include/lfdm_amd.h).
 TIRE_FIELDS = ("xe", "ye", "xedot", "yedot", "omega", "dOmega", "xt", "rStat", "fx", "fy", "fz", "vsx", "vsy",
@@ -236,6 +238,25 @@ class SolverHalfImplicitGPU:
     def set_time(self, t):
         check(lib.lfdm_set_time(self._h, t))
 
+    # -- batching form -----------------------------------------------------------------------
+    def set_horizons(self, steps):
+        """Per-vehicle step counts from t = 0 after which each vehicle stops (None: no horizons)."""
+        if steps is None:
+            check(lib.lfdm_set_horizons(self._h, None))
+            return
+        a = np.ascontiguousarray(steps, dtype=np.int32)
+        if a.shape != (self.total,):
+            raise ValueError(f"horizons must have shape ({self.total},)")
+        check(lib.lfdm_set_horizons(self._h, a.ctypes.data))
+
+    def set_layout(self, layout):
+        check(lib.lfdm_set_layout(self._h, layout))
+
+    def get_layout(self):
+        v = ctypes.c_int()
+        check(lib.lfdm_get_layout(self._h, ctypes.byref(v)))
+        return v.value
+
     # -- differentiation form (device pointers, e.g. torch tensors' data_ptr()) ---------------
     def adjoint_forward(self, nsteps, checkpoint_every, ctrl_ptr, xy_ptr):
         check(lib.lfdm_adjoint_forward(self._h, nsteps, checkpoint_every, ctypes.c_void_p(ctrl_ptr),
@@ -244,6 +265,9 @@ class SolverHalfImplicitGPU:
     def adjoint_backward(self, xy_bar_ptr, ctrl_bar_ptr, state_bar_ptr, param_bar_ptr):
         check(lib.lfdm_adjoint_backward(self._h, ctypes.c_void_p(xy_bar_ptr), ctypes.c_void_p(ctrl_bar_ptr),
                                         ctypes.c_void_p(state_bar_ptr), ctypes.c_void_p(param_bar_ptr)))
+
+    def adjoint_set_tape_budget(self, nbytes):
+        check(lib.lfdm_adjoint_set_tape_budget(self._h, int(nbytes)))
 
     def set_stream(self, stream_ptr):
         check(lib.lfdm_set_stream(self._h, ctypes.c_void_p(stream_ptr)))
@@ -254,4 +278,69 @@ class SolverHalfImplicitGPU:
     def last_launch_ms(self):
         v = ctypes.c_double()
         check(lib.lfdm_last_launch_ms(self._h, ctypes.byref(v)))
+        return v.value
+
+
+class Fleet:
+    """Mixed fleet over solver handles (one bucket per handle, normally one per model), run as one
+    HIP graph per lfdm_fleet_run.  The solvers must outlive the fleet."""
+
+    def __init__(self, solvers=()):
+        f = ctypes.c_void_p()
+        check(lib.lfdm_fleet_create(ctypes.byref(f)))
+        self._f = f
+        self.solvers = []
+        for s in solvers:
+            self.add(s)
+
+    def add(self, solver):
+        idx = check(lib.lfdm_fleet_add(self._f, solver._h))
+        self.solvers.append(solver)
+        return idx
+
+    def close(self):
+        if self._f:
+            lib.lfdm_fleet_destroy(self._f)
+            self._f = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def adjoint_set_tape_budget(self, nbytes):
+        check(lib.lfdm_adjoint_set_tape_budget(self._h, int(nbytes)))
+
+    def set_stream(self, stream_ptr):
+        check(lib.lfdm_fleet_set_stream(self._f, ctypes.c_void_p(stream_ptr)))
+
+    def set_config(self, bucket, block, steps_per_launch=0):
+        check(lib.lfdm_fleet_set_config(self._f, bucket, block, steps_per_launch))
+
+    def get_config(self, bucket):
+        """(block, steps_per_launch, layout) of a bucket."""
+        b, s, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib.lfdm_fleet_get_config(self._f, bucket, ctypes.byref(b), ctypes.byref(s), ctypes.byref(l)))
+        return b.value, s.value, l.value
+
+    def autotune(self, nsteps, trial_steps=1000):
+        check(lib.lfdm_fleet_autotune(self._f, nsteps, trial_steps))
+        return [self.get_config(b) for b in range(len(self.solvers))]
+
+    def run(self, nsteps):
+        check(lib.lfdm_fleet_run(self._f, nsteps))
+
+    def synchronize(self):
+        check(lib.lfdm_fleet_synchronize(self._f))
+
+    def last_ms(self):
+        v = ctypes.c_double()
+        check(lib.lfdm_fleet_last_ms(self._f, ctypes.byref(v)))
         return v.value

@@ -110,7 +110,9 @@ int main(int argc, char** argv) {
     std::vector<int> slot_user(S, -1);
     for (int s = 0; s < n; ++s) slot_user[s] = s;
     const int nseg = (T + K - 1) / K;
-    std::vector<double> ckpt((size_t)nseg * kCarried * S), seg((size_t)K * kSegRow * S), xy((size_t)T * 2 * n);
+    const int G = 3;  // segments per backward group: 10 segments give groups of 3, 3, 3 and 1
+    std::vector<double> ckpt((size_t)nseg * kCarried * S), tape((size_t)G * K * kSegRow * S), xy((size_t)T * 2 * n);
+    std::vector<double> carry((size_t)kCarry * S);
     std::vector<double> xyb((size_t)T * 2 * n), cb((size_t)T * 3 * n), sb((size_t)rec * n, 0.0), pb((size_t)7 * n);
     AdjLaunch L{};
     L.slot_user = slot_user.data();
@@ -119,7 +121,8 @@ int main(int argc, char** argv) {
     L.ctrl = ctrl.data();
     L.xy = xy.data();
     L.ckpt = ckpt.data();
-    L.seg = seg.data();
+    L.tape = tape.data();
+    L.carry = carry.data();
     L.dt = 1e-3;
     L.nsteps = T;
     L.K = K;
@@ -140,7 +143,19 @@ int main(int argc, char** argv) {
     L.ctrl_bar = cb.data();
     L.state_bar = sb.data();
     L.param_bar = pb.data();
-    for (int s = 0; s < S; ++s) adj_backward_lane(L, s, P);
+    // the backward launches' schedule: groups of G segments, last group first; recompute lanes
+    // (q, slot), then one reverse lane per slot with its own constants storage
+    for (int hi = nseg; hi > 0; hi -= G) {
+        L.seg_hi = hi;
+        L.seg_lo = std::max(hi - G, 0);
+        for (int q = 0; q < hi - L.seg_lo; ++q)
+            for (int s = 0; s < S; ++s) adj_recompute_lane(L, q, s, P);
+        for (int s = 0; s < S; ++s) {
+            K18 k;
+            K18b kb;
+            adj_reverse_lane(L, s, P, k, kb);
+        }
+    }
     double worst[4] = {0, 0, 0, 0};
     int hi = 0;
     for (int b = 0; b < nb; ++b) {
