@@ -201,7 +201,8 @@ int lfdm_fleet_create(lfdm_fleet** out);
 int lfdm_fleet_destroy(lfdm_fleet* f);
 /* Adds a bucket; returns its index (>= 0) or a negative LFDM_E_*. */
 int lfdm_fleet_add(lfdm_fleet* f, lfdm_solver* h);
-/* Stream for the fleet's runs (hipStream_t; NULL = the fleet's own). */
+/* Stream for the fleet's runs from now on (hipStream_t; NULL is the default stream).  A fleet whose
+ * caller never chose one uses a non-blocking stream of its own. */
 int lfdm_fleet_set_stream(lfdm_fleet* f, void* stream);
 /* Launch configuration of a bucket: threads per block (64..256, a multiple of 64) and steps per launch. */
 int lfdm_fleet_set_config(lfdm_fleet* f, int bucket, int block, int steps_per_launch);
@@ -215,7 +216,9 @@ int lfdm_fleet_synchronize(lfdm_fleet* f);
 /* Device time of the last lfdm_fleet_run, from HIP events around the graph launch. */
 int lfdm_fleet_last_ms(lfdm_fleet* f, double* ms);
 
-/* Stream used for every launch and copy (hipStream_t; NULL = the handle's own stream). */
+/* Stream used for every launch and copy from now on (hipStream_t).  NULL is the default stream,
+ * which is ordered with every blocking stream (torch's default stream is NULL).  A handle whose
+ * caller never chose a stream uses a non-blocking stream of its own. */
 int lfdm_set_stream(lfdm_solver* h, void* stream);
 int lfdm_synchronize(lfdm_solver* h);
 /* Device time of the last integration launch, from HIP events on the handle's stream. */

@@ -59,10 +59,15 @@ LFDM_HD double sine_step_weight_d(double x) {
 
 LFDM_HD void hypot_adj(double a, double b, double h, double hb, double& ab, double& bb) {
     if (h > 0.0) {
-        ab += hb * a / h;
-        bb += hb * b / h;
+        const double r = hb / h;
+        ab += r * a;
+        bb += r * b;
     }
 }
+
+// The reverse sweeps below multiply by reciprocals of repeated denominators instead of dividing
+// each time (an FP64 division is ~10 instructions): the adjoint's last bits change, the forward
+// values and branch choices it recomputes do not.
 
 // Adjoint of tmxy_combined (dof18.cpp:247-288): bars of (s, df0, sm, fm, ss, fs) from bars of (f, fos).
 LFDM_HD void tmxy_adj(double s, double df0, double sm, double fm, double ss, double fs, double fb, double fosb,
@@ -80,28 +85,30 @@ LFDM_HD void tmxy_adj(double s, double df0, double sm, double fm, double ss, dou
         }
     }
     if (!(s > 0.0 && df0loc > 0.0)) return;
+    const double rs = 1.0 / s, rsm = 1.0 / sm, rfm = 1.0 / fm;
     double dlb = 0.0;
     if (s > ss) {
-        fsb += fb + fosb / s;
-        sb -= fosb * fs / (s * s);
+        fsb += fb + fosb * rs;
+        sb -= fosb * fs * rs * rs;
     } else if (s < sm) {
         const double p = df0loc * sm / fm - 2.0;
         const double sn = s / sm;
         const double dn = 1.0 + (sn + p) * sn;
         const double f = df0loc * sm * sn / dn;
-        dlb += fosb / dn;
-        double dnb = -fosb * df0loc / (dn * dn);
-        dlb += fb * sm * sn / dn;
-        smb += fb * df0loc * sn / dn;
-        double snb = fb * df0loc * sm / dn;
-        dnb -= fb * f / dn;
+        const double rdn = 1.0 / dn;
+        dlb += fosb * rdn;
+        double dnb = -fosb * df0loc * rdn * rdn;
+        dlb += fb * sm * sn * rdn;
+        smb += fb * df0loc * sn * rdn;
+        double snb = fb * df0loc * sm * rdn;
+        dnb -= fb * f * rdn;
         snb += dnb * (2.0 * sn + p);
         const double pb = dnb * sn;
-        sb += snb / sm;
-        smb -= snb * sn / sm;
-        dlb += pb * sm / fm;
-        smb += pb * df0loc / fm;
-        fmb -= pb * df0loc * sm / (fm * fm);
+        sb += snb * rsm;
+        smb -= snb * sn * rsm;
+        dlb += pb * sm * rfm;
+        smb += pb * df0loc * rfm;
+        fmb -= pb * df0loc * sm * rfm * rfm;
     } else {
         const double q = fm / sm;
         const double a = q * q / (df0loc * sm);
@@ -122,8 +129,8 @@ LFDM_HD void tmxy_adj(double s, double df0, double sm, double fm, double ss, dou
             const double sn = (s - sm) / (ss - sm);
             f = fm - (fm - fs) * sn * sn * (3.0 - 2.0 * sn);
         }
-        const double ftb = fb + fosb / s;
-        sb -= fosb * f / (s * s);
+        const double ftb = fb + fosb * rs;
+        sb -= fosb * f * rs * rs;
         if (use_sstar) {
             if (s <= sstar) {
                 const double d = s - sm;
@@ -133,46 +140,49 @@ LFDM_HD void tmxy_adj(double s, double df0, double sm, double fm, double ss, dou
                 smb += ftb * 2.0 * a * d;
             } else {
                 const double g = ss - sstar;
-                const double b = a * (sstar - sm) / g;
+                const double rg = 1.0 / g;
+                const double b = a * (sstar - sm) * rg;
                 const double e = ss - s;
                 fsb += ftb;
                 const double bb = ftb * e * e;
                 ssb += ftb * 2.0 * b * e;
                 sb -= ftb * 2.0 * b * e;
-                ab += bb * (sstar - sm) / g;
-                sstarb += bb * (a / g + a * (sstar - sm) / (g * g));
-                smb -= bb * a / g;
-                ssb -= bb * a * (sstar - sm) / (g * g);
+                ab += bb * (sstar - sm) * rg;
+                sstarb += bb * (a * rg + a * (sstar - sm) * rg * rg);
+                smb -= bb * a * rg;
+                ssb -= bb * a * (sstar - sm) * rg * rg;
             }
             // sstar = sm + (fm - fs) / h, h = a (ss - sm)
+            const double rh = 1.0 / h;
             smb += sstarb;
-            fmb += sstarb / h;
-            fsb -= sstarb / h;
-            const double hb = -sstarb * (fm - fs) / (h * h);
+            fmb += sstarb * rh;
+            fsb -= sstarb * rh;
+            const double hb = -sstarb * (fm - fs) * rh * rh;
             ab += hb * (ss - sm);
             ssb += hb * a;
             smb -= hb * a;
         } else {
             const double w = ss - sm;
-            const double sn = (s - sm) / w;
+            const double rw = 1.0 / w;
+            const double sn = (s - sm) * rw;
             const double g = sn * sn * (3.0 - 2.0 * sn);
             fmb += ftb * (1.0 - g);
             fsb += ftb * g;
             const double snb = -ftb * (fm - fs) * (6.0 * sn - 6.0 * sn * sn);
-            sb += snb / w;
-            smb += snb * (sn - 1.0) / w;
-            ssb -= snb * sn / w;
+            sb += snb * rw;
+            smb += snb * (sn - 1.0) * rw;
+            ssb -= snb * sn * rw;
         }
         // a = fm^2 / (sm^3 df0loc)
-        fmb += ab * 2.0 * a / fm;
-        smb -= ab * 3.0 * a / sm;
+        fmb += ab * 2.0 * a * rfm;
+        smb -= ab * 3.0 * a * rsm;
         dlb -= ab * a / df0loc;
     }
     if (which == 1) {
         df0b += dlb;
     } else if (which == 2) {
-        fmb += dlb * 2.0 / sm;
-        smb -= dlb * 2.0 * fm / (sm * sm);
+        fmb += dlb * 2.0 * rsm;
+        smb -= dlb * 2.0 * fm * rsm * rsm;
     }
 }
 
@@ -246,6 +256,7 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     const double wxw = sine_step_weight(avx), wyw = sine_step_weight(avy);
 
     // ---- reverse ----
+    const double rhsxn = 1.0 / hsxn, rhsyn = 1.0 / hsyn, rvta = 1.0 / vta;
     double vsx2b = 0, vyb = 0, xedotb = B.xedot, yedotb = B.yedot, xeb = 0, yeb = 0;
     // fx = wxw fxstr + (1 - wxw) fxdyn
     const double wxwb = B.fx * (fxstr - fxdyn), fxstrb = B.fx * wxw, fxdynb = B.fx * (1. - wxw);
@@ -259,8 +270,8 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     yeb += fyrb * t.cy + fydynb * t.cy;
     yedotb += fyrb * t.dy + fydynb * t.dy;
     // xedot = numx / denx, yedot = numy / deny
-    const double numxb = xedotb / denx, denxb = -xedotb * xedot / denx;
-    const double numyb = yedotb / deny, denyb = -yedotb * yedot / deny;
+    const double numxb = xedotb / denx, denxb = -numxb * xedot;
+    const double numyb = yedotb / deny, denyb = -numyb * yedot;
     double vtxsb = -numxb * t.cx * xe + denxb * t.dx;
     xeb += -numxb * vtxs * t.cx;
     double fosb = -numxb * vsx2 + denxb;
@@ -290,14 +301,16 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     double cab = 0, sab = 0;
     double dfx0b = 0, dfy0b = 0, fxmb = 0, fymb = 0, fxsb = 0, fysb = 0, sxmb = 0, symb = 0, sxsb = 0, sysb = 0;
     // I = sxs ca / hsxn, J = sys sa / hsyn
-    sxsb += Ib * ca / hsxn; cab += Ib * c.sxs / hsxn; hsxnb -= Ib * I / hsxn;
-    sysb += Jb * sa / hsyn; sab += Jb * c.sys / hsyn; hsynb -= Jb * J / hsyn;
+    const double Ibr = Ib * rhsxn, Jbr = Jb * rhsyn;
+    sxsb += Ibr * ca; cab += Ibr * c.sxs; hsxnb -= Ibr * I;
+    sysb += Jbr * sa; sab += Jbr * c.sys; hsynb -= Jbr * J;
     // G = fxs ca, H = fys sa
     fxsb += Gb * ca; cab += Gb * c.fxs;
     fysb += Hb * sa; sab += Hb * c.fys;
     // E = sxm ca / hsxn, F = sym sa / hsyn
-    sxmb += Eb * ca / hsxn; cab += Eb * c.sxm / hsxn; hsxnb -= Eb * E / hsxn;
-    symb += Fb * sa / hsyn; sab += Fb * c.sym / hsyn; hsynb -= Fb * F / hsyn;
+    const double Ebr = Eb * rhsxn, Fbr = Fb * rhsyn;
+    sxmb += Ebr * ca; cab += Ebr * c.sxm; hsxnb -= Ebr * E;
+    symb += Fbr * sa; sab += Fbr * c.sym; hsynb -= Fbr * F;
     // C = fxm ca, D = fym sa
     fxmb += Cb * ca; cab += Cb * c.fxm;
     fymb += Db * sa; sab += Db * c.fym;
@@ -306,34 +319,35 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     dfy0b += Bb * sa * hsyn; sab += Bb * c.dfy0 * hsyn; hsynb += Bb * c.dfy0 * sa;
     double sxnb = 0, synb = 0;
     if (sc > 0) {
-        // ca = sxn / sc, sa = syn / sc
-        sxnb += cab / sc;
-        synb += sab / sc;
-        scb -= (cab * ca + sab * sa) / sc;
-        // sc = hypot(sxn, syn)
-        sxnb += scb * sxn / sc;
-        synb += scb * syn / sc;
+        // ca = sxn / sc, sa = syn / sc, sc = hypot(sxn, syn)
+        const double rsc = 1.0 / sc;
+        scb -= (cab * ca + sab * sa) * rsc;
+        sxnb += (cab + scb * sxn) * rsc;
+        synb += (sab + scb * syn) * rsc;
     }
     // sxn = sx / hsxn, syn = sy / hsyn
-    const double sxb = sxnb / hsxn;
-    hsxnb -= sxnb * sxn / hsxn;
-    syb += synb / hsyn;
-    hsynb -= synb * syn / hsyn;
+    const double sxnbr = sxnb * rhsxn, synbr = synb * rhsyn;
+    const double sxb = sxnbr;
+    hsxnb -= sxnbr * sxn;
+    syb += synbr;
+    hsynb -= synbr * syn;
     // hsxn = sxm/ssum + rx/rsum, hsyn = sym/ssum + ry/rsum
-    sxmb += hsxnb / ssum;
-    symb += hsynb / ssum;
-    const double ssumb = -(hsxnb * c.sxm + hsynb * c.sym) / (ssum * ssum);
-    double rxb = hsxnb / rsum, ryb = hsynb / rsum;
-    const double rsumb = -(hsxnb * rx + hsynb * ry) / (rsum * rsum);
+    const double rssum = 1.0 / ssum, rrsum = 1.0 / rsum;
+    sxmb += hsxnb * rssum;
+    symb += hsynb * rssum;
+    const double ssumb = -(hsxnb * c.sxm + hsynb * c.sym) * rssum * rssum;
+    double rxb = hsxnb * rrsum, ryb = hsynb * rrsum;
+    const double rsumb = -(hsxnb * rx + hsynb * ry) * rrsum * rrsum;
     rxb += rsumb;
     ryb += rsumb;
     sxmb += ssumb;
     symb += ssumb;
     // rx = fxm / dfx0, ry = fym / dfy0
-    fxmb += rxb / c.dfx0;
-    dfx0b -= rxb * rx / c.dfx0;
-    fymb += ryb / c.dfy0;
-    dfy0b -= ryb * ry / c.dfy0;
+    const double rxbr = rxb / c.dfx0, rybr = ryb / c.dfy0;
+    fxmb += rxbr;
+    dfx0b -= rxbr * rx;
+    fymb += rybr;
+    dfy0b -= rybr * ry;
     // curve_at(fzc): InterpQ v = fzn (q1 - q2 fzn), InterpL v = w1 + l (fzn - 1)
     double fznb = dfx0b * (u.qx1 - 2.0 * u.qx2 * fzn) + dfy0b * (qy1 - 2.0 * qy2 * fzn) +
                   fxmb * (u.qfxm1 - 2.0 * u.qfxm2 * fzn) + fymb * (u.qfym1 - 2.0 * u.qfym2 * fzn) +
@@ -341,16 +355,16 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
                   symb * u.lsym + sxsb * u.lsxs + sysb * u.lsys;
     B.qy1 += dfy0b * fzn;
     B.qy2 -= dfy0b * fzn * fzn;
-    if (!clip) fzb += fznb / t.pn;
+    if (!clip) fzb += fznb * u.inv_pn;
     // sy = -tan(alpha), alpha = atan2(vsy, vta) - delta
     const double alphab = -syb * (1.0 + ta * ta);
     B.delta -= alphab;
-    const double r2 = vsy * vsy + vta * vta;
-    B.vsy += alphab * vta / r2;
-    vtab -= alphab * vsy / r2;
+    const double ar2 = alphab / (vsy * vsy + vta * vta);
+    B.vsy += ar2 * vta;
+    vtab -= ar2 * vsy;
     // sx = -vsx2 / vta
-    vsx2b -= sxb / vta;
-    vtab += sxb * vsx2 / (vta * vta);
+    vsx2b -= sxb * rvta;
+    vtab += sxb * vsx2 * rvta * rvta;
     // vta = rw + 0.01, rw = r_eff |omega|
     rwb += vtab;
     double r_effb = rwb * aom;
@@ -364,7 +378,7 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     rStatb += r_effb * (1. - rdynco);
     fzb += rdyncob * drd;
     // rStat = r0 - fz / kt
-    fzb -= rStatb / t.kt;
+    fzb -= rStatb * u.inv_kt;
     B.fz += fzb;
     B.xe += xeb;
     B.ye += yeb;
@@ -490,9 +504,10 @@ LFDM_HD void powertrain4_adj(const PSet& P, const Derived& u, double tqs, double
     if (clip) ttb = 0;
     // drive torque at w_dt: tqs * T(w) * throttle + L(w)
     const double Tw = map_y(P.torque, w_dt), Ts = map_slope(P.torque, w_dt), Ls = map_slope(P.losses, w_dt);
+    const double rgr = 1.0 / gr;
     if (P.tcbool) {
         const double omega_in = crank;
-        double toutb = ttb / gr;
+        double toutb = ttb * rgr;
         double tinb = rev ? -toutb : -tr * toutb;
         const double trb = rev ? 0.0 : -toutb * torque_in;
         // dcrank = inv_ci (DT + torque_in)
@@ -503,25 +518,27 @@ LFDM_HD void powertrain4_adj(const PSet& P, const Derived& u, double tqs, double
         B.crank += DTb * (tqs * throttle * Ts + Ls);
         // torque_in = -q^2 (negated when reversed)
         const double qb = tinb * (rev ? 2.0 * q : -2.0 * q);
-        double omega_inb = qb / cf;
-        const double cfb = -qb * q / cf;
+        const double qbr = qb / cf;
+        double omega_inb = qbr;
+        const double cfb = -qbr * q;
         const double srb = cfb * cfs + trb * trs;
         if (active) {
             const double sr0b = rev ? -srb : srb;
             const double omega_outb = sr0b / omega_in;
-            omega_inb -= sr0b * sr0 / omega_in;
+            omega_inb -= omega_outb * sr0;
+            const double ob = omega_outb * 0.25 * rgr;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) B.omega[i] += omega_outb * 0.25 / gr;
+            for (int i = 0; i < 4; ++i) B.omega[i] += ob;
         }
         B.crank += omega_inb;
     } else {
         // crank_out = w_dt = 0.25 osum / gr; torque_t = DT(w_dt) / gr
-        const double DTb = ttb / gr;
+        const double DTb = ttb * rgr;
         B.tqs += DTb * Tw * throttle;
         B.throttle += DTb * tqs * Tw;
-        const double wb = B.crank_out + DTb * (tqs * throttle * Ts + Ls);
+        const double wb = (B.crank_out + DTb * (tqs * throttle * Ts + Ls)) * 0.25 * rgr;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) B.omega[i] += wb * 0.25 / gr;
+        for (int i = 0; i < 4; ++i) B.omega[i] += wb;
     }
 }
 
@@ -674,8 +691,9 @@ LFDM_HD void step18_adj(const PSet& P, const Derived& u, const K18& k, const S18
         omb[i] = o.omega[i];
     }
     // ---- chassis (computeVehRHS) ----
-    const double Nvb = vdotb / k.denom, Nxb = wxdotb / k.denom, Nzb = wzdotb / k.denom;
-    kb.denom -= (vdotb * vdot + wxdotb * wxdot + wzdotb * wzdot) / k.denom;
+    const double rden = 1.0 / k.denom;
+    const double Nvb = vdotb * rden, Nxb = wxdotb * rden, Nzb = wzdotb * rden;
+    kb.denom -= (Nvb * vdot + Nxb * wxdot + Nzb * wzdot);
     double E1b = Nvb * (u.jxz2 - A2 * jz) + Nxb * (A1 * jxz - A3 * jz) + Nzb * (-k.A1A2 + A3 * jxz);
     double E2b = Nvb * (-k.A1A2 + A3 * jxz) + Nxb * (-k.A1A3 + jxz * mt) + Nzb * (k.A3sq - A2 * mt);
     double E3b = Nvb * (A1 * jxz - A3 * jz) + Nxb * (u.A1sq - jz * mt) + Nzb * (-k.A1A3 + jxz * mt);
@@ -803,10 +821,10 @@ LFDM_HD void step18_adj(const PSet& P, const Derived& u, const K18& k, const S18
     kb.Z2fc += Z2fb * lat;
     kb.Z2rc += Z2rb * lat;
     const double latb = Z2fb * k.Z2fc + Z2rb * k.Z2rc;
-    sb.phi += Z3fb * P.krof / P.cf + Z3rb * P.kror / P.cr;
-    sb.wx += Z3fb * P.brof / P.cf + Z3rb * P.bror / P.cr;
-    kb.Z4n += Z4b * (s0.udot - s0.wz * s0.v) / u.two_ab;
-    const double ddb = Z4b * k.Z4n / u.two_ab;
+    sb.phi += Z3fb * u.krof_cf + Z3rb * u.kror_cr;
+    sb.wx += Z3fb * u.brof_cf + Z3rb * u.bror_cr;
+    kb.Z4n += Z4b * (s0.udot - s0.wz * s0.v) * u.inv_two_ab;
+    const double ddb = Z4b * k.Z4n * u.inv_two_ab;
     sb.udot += ddb;
     sb.wz -= ddb * s0.v;
     sb.v -= ddb * s0.wz;

@@ -24,6 +24,11 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Waves per SIMD of the segment recompute (two: the group's (segment, vehicle) lanes fill them).
+#ifndef LFDM_ADJ_RECOMPUTE_WAVES
+#define LFDM_ADJ_RECOMPUTE_WAVES 2
+#endif
+
 static_assert(sizeof(K18) % 16 == 8 && sizeof(K18b) % 16 == 8, "odd number of doubles per lane in LDS");
 
 template <int W>
@@ -34,7 +39,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W, W))
     adj_forward_lane(L, slot, L.sets[uniform(L.wave_set[wave])]);
 }
 
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void adj_recompute(AdjLaunch L) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(LFDM_ADJ_RECOMPUTE_WAVES,
+                                                                            LFDM_ADJ_RECOMPUTE_WAVES))) void
+adj_recompute(AdjLaunch L) {
     const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     // S is a multiple of the wavefront size, so q and the slot's wave are wave-uniform
     const int q = uniform((int)(id / (size_t)L.S));

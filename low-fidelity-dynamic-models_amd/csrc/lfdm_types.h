@@ -42,6 +42,8 @@ struct Derived {
     double qx1, qx2, qfxm1, qfxm2, qfym1, qfym2, qfxs1, qfxs2, qfys1, qfys2;  // InterpQ pairs (2w1 - w2/2, w1 - w2/2)
     double lsxm, lsym, lsxs, lsys, lrd;                                       // InterpL slopes (w2 - w1)
     double delta_rear;  // steering angle of the rear tires (steering input 0)
+    // reciprocals and quotients only the 18-DOF step adjoint reads
+    double inv_pn, inv_kt, inv_two_ab, krof_cf, kror_cr, brof_cf, bror_cr;
 };
 
 // Vehicle parameters of one set (11-DOF dof11.h:496-620, 18-DOF dof18.h:489-799, 24-DOF

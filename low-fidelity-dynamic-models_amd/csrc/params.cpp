@@ -397,6 +397,13 @@ void derive18(PSet& p) {
     u.lsys = t.sysP2n - t.sysPn;
     u.lrd = t.rdyncoP2n - t.rdyncoPn;
     u.delta_rear = p.nonLinearSteer ? host_map_y(p.steer, 0.0) : 0.0 * p.maxSteer;
+    u.inv_pn = 1. / t.pn;
+    u.inv_kt = 1. / t.kt;
+    u.inv_two_ab = 1. / u.two_ab;
+    u.krof_cf = p.krof / p.cf;
+    u.kror_cr = p.kror / p.cr;
+    u.brof_cf = p.brof / p.cf;
+    u.bror_cr = p.bror / p.cr;
 }
 
 }  // namespace

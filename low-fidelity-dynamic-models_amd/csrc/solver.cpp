@@ -117,6 +117,7 @@ struct lfdm_solver {
     std::vector<int> user_slot;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    bool stream_set = false;  // the caller chose `stream` (NULL = the default stream)
     PSet* d_sets = nullptr;
     int* d_wave_set = nullptr;
     int* d_slot_user = nullptr;
@@ -163,6 +164,7 @@ struct lfdm_fleet {
     std::vector<Cfg> cfg;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    bool stream_set = false;  // the caller chose `stream` (NULL = the default stream)
     std::vector<hipStream_t> branch;
     std::vector<hipEvent_t> pre, join;
     hipEvent_t fork = nullptr, post = nullptr, ev0 = nullptr, ev1 = nullptr;
@@ -188,11 +190,13 @@ int ensure_buffer(double*& p, size_t& cap, size_t n) {
 }
 
 int ensure_stream(lfdm_solver* h) {
-    if (h->stream == nullptr && !h->own_stream) {
+    if (!h->ev0) {
         int ndev = 0;
         hipError_t e = hipGetDeviceCount(&ndev);
         if (e != hipSuccess || ndev == 0)
             return fail(LFDM_E_HIP, "no HIP device available: the solver runs only on an AMD GPU");
+    }
+    if (!h->stream_set && !h->own_stream) {
         HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         h->own_stream = true;
     }
@@ -614,7 +618,7 @@ int lfdm_create(int dof, unsigned total, lfdm_solver** out) {
 
 int lfdm_destroy(lfdm_solver* h) {
     if (!h) return LFDM_OK;
-    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->stream_set || h->own_stream) (void)hipStreamSynchronize(h->stream);
     dfree(h->d_sets);
     dfree(h->d_wave_set);
     dfree(h->d_slot_user);
@@ -1216,14 +1220,14 @@ int lfdm_set_stream(lfdm_solver* h, void* stream) {
         h->own_stream = false;
     }
     h->stream = (hipStream_t)stream;
-    if (!stream) return LFDM_OK;
     h->own_stream = false;
+    h->stream_set = true;
     return LFDM_OK;
 }
 
 int lfdm_synchronize(lfdm_solver* h) {
     if (int rc = check(h)) return rc;
-    if (!h->stream && !h->own_stream) return LFDM_OK;
+    if (!h->stream_set && !h->own_stream) return LFDM_OK;  // nothing launched yet
     HIPCHK(hipStreamSynchronize(h->stream));
     return LFDM_OK;
 }
@@ -1264,7 +1268,7 @@ int fleet_prepare(lfdm_fleet* f, int nsteps) {
         if (int rc = sync_inputs(h)) return rc;
         if (int rc = check_controls(h, nsteps)) return rc;
     }
-    if (!f->stream) {
+    if (!f->stream_set && !f->own_stream) {
         HIPCHK(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
         f->own_stream = true;
     }
@@ -1321,7 +1325,7 @@ int lfdm_fleet_create(lfdm_fleet** out) {
 
 int lfdm_fleet_destroy(lfdm_fleet* f) {
     if (!f) return LFDM_OK;
-    if (f->stream) (void)hipStreamSynchronize(f->stream);
+    if (f->stream_set || f->own_stream) (void)hipStreamSynchronize(f->stream);
     fleet_drop_graph(f);
     for (hipStream_t st : f->branch) (void)hipStreamDestroy(st);
     for (hipEvent_t e : f->pre) (void)hipEventDestroy(e);
@@ -1352,6 +1356,7 @@ int lfdm_fleet_set_stream(lfdm_fleet* f, void* stream) {
     }
     f->stream = (hipStream_t)stream;
     f->own_stream = false;
+    f->stream_set = true;
     return LFDM_OK;
 }
 
@@ -1484,7 +1489,7 @@ int lfdm_fleet_autotune(lfdm_fleet* f, int nsteps, int trial_steps) {
 
 int lfdm_fleet_synchronize(lfdm_fleet* f) {
     if (int rc = fleet_check(f)) return rc;
-    if (f->stream) HIPCHK(hipStreamSynchronize(f->stream));
+    if (f->stream_set || f->own_stream) HIPCHK(hipStreamSynchronize(f->stream));
     return LFDM_OK;
 }
 

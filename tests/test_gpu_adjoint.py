@@ -188,3 +188,27 @@ def test_torch_rollout_backward_and_mpc_step():
     say(f"[adjoint] MPC mean loss per vehicle: {[round(h, 4) for h in hist]}")
     assert hist[-1] < hist[0]
     s.close()
+
+
+def test_default_stream_orders_with_torch():
+    """A handle handed torch's default stream (NULL) before its first launch launches on it: torch work
+    queued after the adjoint forward sees the finished trajectory without an explicit synchronize,
+    and the fill queued before it cannot land after it."""
+    from lfdm_amd.mpc import arc_reference, initial_controls, moving_start
+    from lfdm_amd.torch_ops import default_params, make_solver
+
+    flat = hmmwv()
+    n, T = 8192, 2000
+    s = make_solver(flat, n)
+    s.set_stream(torch.cuda.current_stream().cuda_stream)
+    s.set_overrides(default_params(flat, n).numpy())
+    s.set_states(moving_start(flat, n).numpy())
+    _, kappa = arc_reference(n, T, seed=5)
+    ctrl = initial_controls(flat, n, T, kappa)
+    xy = torch.full((T, 2, n), float("nan"), dtype=torch.float64, device="cuda")
+    s.adjoint_forward(T, 100, ctrl.data_ptr(), xy.data_ptr())
+    early = xy.clone()  # queued on torch's stream right after the launch
+    torch.cuda.synchronize()
+    assert torch.isfinite(xy).all()
+    assert torch.equal(early, xy)
+    s.close()
