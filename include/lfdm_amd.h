@@ -165,10 +165,11 @@ int lfdm_adjoint_forward(lfdm_solver* h, int nsteps, int checkpoint_every, const
                          double* device_xy);
 int lfdm_adjoint_backward(lfdm_solver* h, const double* device_xy_bar, double* device_ctrl_bar,
                           double* device_state_bar, double* device_param_bar);
-/* Device memory the backward may use for its tape (default 16 GiB).  The backward walks the
- * checkpoint segments in groups, recomputing a whole group's steps at once (56 doubles per vehicle
- * and step); a group holds as many segments as fill the GPU, within this budget, and at least one.
- * Results do not depend on it. */
+/* Device memory the adjoint may use for its tape (default 16 GiB), from the next
+ * lfdm_adjoint_forward on.  The backward walks the checkpoint segments in groups, recomputing a whole
+ * group's steps at once (38 doubles per vehicle and step; the forward tapes the last group itself);
+ * a group holds as many segments as fill the GPU, within this budget, and at least one.  Results do
+ * not depend on it. */
 int lfdm_adjoint_set_tape_budget(lfdm_solver* h, unsigned long long bytes);
 
 /* ---- batching form: per-vehicle horizons, layouts, mixed fleets (no counterpart in the

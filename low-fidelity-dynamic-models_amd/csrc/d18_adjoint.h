@@ -50,6 +50,22 @@ LFDM_HD double map_slope(const Map& m, double x) {
     return (m.e[r].y - m.e[r - 1].y) / (m.e[r].x - m.e[r - 1].x);
 }
 
+// map_y(m, x) and the slope of the segment it uses (0 where it clamps), from one scan of the knots.
+LFDM_HD double map_y_slope(const Map& m, double x, double& slope) {
+    const int n = m.n;
+    slope = 0.0;
+    if (x <= m.e[0].x) return m.e[0].y;
+    if (x >= m.e[n - 1].x) return m.e[n - 1].y;
+    int r = 0;
+    for (int k = 1; k < n - 1; ++k) r += (m.e[k].x < x) ? 1 : 0;
+    r += 1;
+    const MapXY L = m.e[r - 1];
+    const MapXY R = m.e[r];
+    slope = (R.y - L.y) / (R.x - L.x);
+    const double mbar = (x - L.x) / (R.x - L.x);
+    return L.y + mbar * (R.y - L.y);
+}
+
 // d sineStep(x, 1, 1, 1.5, 0) / dx
 LFDM_HD double sine_step_weight_d(double x) {
     if (x <= 1.0 || x >= 1.5) return 0.0;
@@ -448,10 +464,8 @@ LFDM_HD void powertrain4_adj(const PSet& P, const Derived& u, double tqs, double
                 active = false;
             }
         }
-        cf = map_y(P.cfmap, sr);
-        tr = map_y(P.trmap, sr);
-        cfs = map_slope(P.cfmap, sr);
-        trs = map_slope(P.trmap, sr);
+        cf = map_y_slope(P.cfmap, sr, cfs);
+        tr = map_y_slope(P.trmap, sr, trs);
         q = omega_in / cf;
         torque_in = -(q * q);
         if (rev) torque_in = -torque_in;
@@ -503,7 +517,8 @@ LFDM_HD void powertrain4_adj(const PSet& P, const Derived& u, double tqs, double
         ttb = tfb;
     if (clip) ttb = 0;
     // drive torque at w_dt: tqs * T(w) * throttle + L(w)
-    const double Tw = map_y(P.torque, w_dt), Ts = map_slope(P.torque, w_dt), Ls = map_slope(P.losses, w_dt);
+    double Ts;
+    const double Tw = map_y_slope(P.torque, w_dt, Ts), Ls = map_slope(P.losses, w_dt);
     const double rgr = 1.0 / gr;
     if (P.tcbool) {
         const double omega_in = crank;

@@ -4,7 +4,8 @@
 //  - adj_recompute: one lane per (segment of a group, vehicle slot); reruns the segment from its
 //    checkpoint into the tape.  Segments are independent, so a group of G segments gives G times
 //    the wavefronts of the fleet and the recompute runs at full occupancy even for fleets that fill
-//    only one wavefront per SIMD.
+//    only one wavefront per SIMD.  The tape is blocked by wavefront (one contiguous 19 KB run per
+//    wavefront and step) and the per-vehicle constants sit in LDS.
 //  - adj_reverse: one lane per slot; walks the group's segments backwards through the step adjoint
 //    of d18_adjoint.h.  The step adjoint needs the whole register file of a SIMD, so the per-vehicle
 //    constants and their adjoints live in LDS (23 + 21 doubles per lane; both structs have an odd
@@ -47,7 +48,8 @@ adj_recompute(AdjLaunch L) {
     const int q = uniform((int)(id / (size_t)L.S));
     const int slot = (int)(id - (size_t)q * L.S);
     if (q >= L.seg_hi - L.seg_lo) return;
-    adj_recompute_lane(L, q, slot, L.sets[uniform(L.wave_set[slot >> 6])]);
+    __shared__ K18 ks[kThreads];
+    adj_recompute_lane(L, q, slot, L.sets[uniform(L.wave_set[slot >> 6])], ks[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void adj_reverse(AdjLaunch L) {

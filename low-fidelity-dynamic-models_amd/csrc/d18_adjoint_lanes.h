@@ -7,31 +7,59 @@
 namespace lfdm {
 namespace dev {
 
-constexpr int kCarried = 32;  // packed S18: xe ye xedot yedot omega (x4), 11 chassis fields, gear
-constexpr int kTireOut = 24;  // packed TireOut18: fx fy My rStat xedot yedot (x4)
-constexpr int kSegRow = kCarried + kTireOut;  // one step of the segment buffer
+constexpr int kCarried = 32;  // checkpoint: packed S18, xe ye xedot yedot omega (x4), 11 chassis fields, gear
+// One step of the tape: what the step adjoint reads when it is given the step's tire outputs.  It
+// never reads x, y or the tire deflection rates (they enter the step only through updates whose
+// adjoints need no values), so the tape leaves them out.
+constexpr int kTapeState = 22;  // xe ye omega (x4), u v psi wz phi wx udot vdot crank, gear
+constexpr int kTapeTires = 16;  // fx fy My rStat (x4)
+constexpr int kSegRow = kTapeState + kTapeTires;
 
-LFDM_HD void pack_to(const TireOut18& t, double* p, size_t S) {
+LFDM_HD void tape_put_state(const S18& s, double* p, size_t st) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        p[(0 + i) * S] = t.fx[i];
-        p[(4 + i) * S] = t.fy[i];
-        p[(8 + i) * S] = t.My[i];
-        p[(12 + i) * S] = t.rStat[i];
-        p[(16 + i) * S] = t.xedot[i];
-        p[(20 + i) * S] = t.yedot[i];
+        p[(0 + i) * st] = s.xe[i];
+        p[(4 + i) * st] = s.ye[i];
+        p[(8 + i) * st] = s.omega[i];
+    }
+    const double v[9] = {s.u, s.v, s.psi, s.wz, s.phi, s.wx, s.udot, s.vdot, s.crank};
+#pragma unroll
+    for (int j = 0; j < 9; ++j) p[(12 + j) * st] = v[j];
+    p[21 * st] = (double)s.gear;
+}
+
+LFDM_HD void tape_put_tires(const TireOut18& t, double* p, size_t st) {
+    p += (size_t)kTapeState * st;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        p[(0 + i) * st] = t.fx[i];
+        p[(4 + i) * st] = t.fy[i];
+        p[(8 + i) * st] = t.My[i];
+        p[(12 + i) * st] = t.rStat[i];
     }
 }
 
-LFDM_HD void unpack_to(const double* p, size_t S, TireOut18& t) {
+// Reads one tape step; the fields the tape leaves out are set to zero.
+LFDM_HD void tape_get(const double* p, size_t st, S18& s, TireOut18& t) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        t.fx[i] = p[(0 + i) * S];
-        t.fy[i] = p[(4 + i) * S];
-        t.My[i] = p[(8 + i) * S];
-        t.rStat[i] = p[(12 + i) * S];
-        t.xedot[i] = p[(16 + i) * S];
-        t.yedot[i] = p[(20 + i) * S];
+        s.xe[i] = p[(0 + i) * st];
+        s.ye[i] = p[(4 + i) * st];
+        s.omega[i] = p[(8 + i) * st];
+        s.xedot[i] = s.yedot[i] = 0.0;
+    }
+    s.x = s.y = 0.0;
+    s.u = p[12 * st]; s.v = p[13 * st]; s.psi = p[14 * st]; s.wz = p[15 * st]; s.phi = p[16 * st];
+    s.wx = p[17 * st]; s.udot = p[18 * st]; s.vdot = p[19 * st]; s.crank = p[20 * st];
+    s.gear = (int)p[21 * st];
+    const double* q = p + (size_t)kTapeState * st;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        t.fx[i] = q[(0 + i) * st];
+        t.fy[i] = q[(4 + i) * st];
+        t.My[i] = q[(8 + i) * st];
+        t.rStat[i] = q[(12 + i) * st];
+        t.xedot[i] = t.yedot[i] = 0.0;
     }
 }
 
@@ -84,6 +112,17 @@ LFDM_HD void load_k(const PSet& P, const double* o, size_t S, K18& k) {
     k18_init(P, o[0], o[S], o[2 * S], o[3 * S], o[4 * S], o[5 * S], o[6 * S], k);
 }
 
+// Tape of a backward group, blocked by wavefront: [q][wave][K][kSegRow][64], so the rows a
+// wavefront writes (and later reads) for one step are one contiguous run of kSegRow x 512 bytes.
+// Returns the lane's field-0 entry of step jj of tape block q; fields are 64 doubles apart.
+LFDM_HD double* tape_row(const AdjLaunch& L, int q, int jj, int slot) {
+    const size_t wave = (size_t)q * (size_t)(L.S / kWave) + (size_t)(slot / kWave);
+    return L.tape + ((wave * (size_t)L.K + (size_t)jj) * kSegRow) * kWave + (size_t)(slot % kWave);
+}
+
+// Forward pass of one slot: checkpoints every K steps and (x, y) after every step.  The steps of
+// segments fwd_tape_lo and later (the group the backward walks first) also go to the tape, so the
+// backward needs no recompute for that group.
 LFDM_HD void adj_forward_lane(const AdjLaunch& L, int slot, const PSet& P) {
     const size_t S = (size_t)L.S, N = (size_t)L.n_user;
     S18 s;
@@ -92,10 +131,24 @@ LFDM_HD void adj_forward_lane(const AdjLaunch& L, int slot, const PSet& P) {
     load_k(P, L.ovr + slot, S, k);
     const int user = L.slot_user[slot];
     const int uv = user >= 0 ? user : 0;
-    for (int j = 0; j < L.nsteps; ++j) {
+    const int jt = L.fwd_tape_lo * L.K < L.nsteps ? L.fwd_tape_lo * L.K : L.nsteps;
+    for (int j = 0; j < jt; ++j) {
         if (j % L.K == 0) pack(s, L.ckpt + (size_t)(j / L.K) * kCarried * S + slot, S);
         const double* c = L.ctrl + (size_t)j * 3 * N + uv;
         step18<false, false>(P, P.d, k, s, c[0], c[N], c[2 * N], L.dt, nullptr);
+        if (user >= 0) {
+            L.xy[(size_t)j * 2 * N + user] = s.x;
+            L.xy[((size_t)j * 2 + 1) * N + user] = s.y;
+        }
+    }
+    for (int j = jt; j < L.nsteps; ++j) {
+        if (j % L.K == 0) pack(s, L.ckpt + (size_t)(j / L.K) * kCarried * S + slot, S);
+        double* row = tape_row(L, j / L.K - L.fwd_tape_lo, j % L.K, slot);
+        tape_put_state(s, row, kWave);
+        const double* c = L.ctrl + (size_t)j * 3 * N + uv;
+        TireOut18 to;
+        step18<false, false, true>(P, P.d, k, s, c[0], c[N], c[2 * N], L.dt, nullptr, &to);
+        tape_put_tires(to, row, kWave);
         if (user >= 0) {
             L.xy[(size_t)j * 2 * N + user] = s.x;
             L.xy[((size_t)j * 2 + 1) * N + user] = s.y;
@@ -145,19 +198,15 @@ LFDM_HD void load_carry(const double* p, size_t S, S18b& l, K18b& kb) {
     kb.jz = v[29]; kb.tqs = v[30]; kb.maxBrake = v[31];
 }
 
-// Keeps the compiler from carrying values read from memory across this point in registers: the
-// reverse lane keeps the per-vehicle constants and their adjoints in memory (LDS on the GPU) and
-// reads them where each step uses them, which frees ~90 registers of the reverse step.
-#define LFDM_RELOAD_MEMORY() asm volatile("" ::: "memory")
 
 // Segment recompute of the backward pass, one lane per (segment seg_lo + q, slot): from the
 // segment's checkpoint, reruns its steps and writes each step's carried state and tire outputs to
-// tape block q ([q][K][kSegRow][S]).  Segments are independent, so a launch covers a whole group.
-LFDM_HD void adj_recompute_lane(const AdjLaunch& L, int q, int slot, const PSet& P) {
+// tape block q.  Segments are independent, so a launch covers a whole group.  k is the lane's
+// storage for its per-vehicle constants (LDS on the GPU, read where each step uses them).
+LFDM_HD void adj_recompute_lane(const AdjLaunch& L, int q, int slot, const PSet& P, K18& k) {
     const int seg = L.seg_lo + q;
     if (seg >= L.seg_hi) return;
     const size_t S = (size_t)L.S, N = (size_t)L.n_user;
-    K18 k;
     load_k(P, L.ovr + slot, S, k);
     S18 s;
     unpack(L.ckpt + (size_t)seg * kCarried * S + slot, S, s);
@@ -165,14 +214,14 @@ LFDM_HD void adj_recompute_lane(const AdjLaunch& L, int q, int slot, const PSet&
     const int uv = user >= 0 ? user : 0;
     const int j0 = seg * L.K;
     const int j1 = (L.nsteps < j0 + L.K ? L.nsteps : j0 + L.K);
-    double* blk = L.tape + (size_t)q * L.K * kSegRow * S + slot;
     for (int j = j0; j < j1; ++j) {
-        double* row = blk + (size_t)(j - j0) * kSegRow * S;
-        pack(s, row, S);
+        LFDM_RELOAD_MEMORY();
+        double* row = tape_row(L, q, j - j0, slot);
+        tape_put_state(s, row, kWave);
         const double* c = L.ctrl + (size_t)j * 3 * N + uv;
         TireOut18 to;
         step18<false, false, true>(P, P.d, k, s, c[0], c[N], c[2 * N], L.dt, nullptr, &to);
-        pack_to(to, row + (size_t)kCarried * S, S);
+        tape_put_tires(to, row, kWave);
     }
 }
 
@@ -196,14 +245,11 @@ LFDM_HD void adj_reverse_lane(const AdjLaunch& L, int slot, const PSet& P, K18& 
     for (int seg = L.seg_hi - 1; seg >= L.seg_lo; --seg) {
         const int j0 = seg * L.K;
         const int j1 = (L.nsteps < j0 + L.K ? L.nsteps : j0 + L.K);
-        const double* blk = L.tape + (size_t)(seg - L.seg_lo) * L.K * kSegRow * S + slot;
         for (int j = j1 - 1; j >= j0; --j) {
             LFDM_RELOAD_MEMORY();
-            const double* row = blk + (size_t)(j - j0) * kSegRow * S;
             S18 s;
-            unpack(row, S, s);
             TireOut18 to;
-            unpack_to(row + (size_t)kCarried * S, S, to);
+            tape_get(tape_row(L, seg - L.seg_lo, j - j0, slot), kWave, s, to);
             lam.x += L.xy_bar[(size_t)j * 2 * N + uv];
             lam.y += L.xy_bar[((size_t)j * 2 + 1) * N + uv];
             const double* c = L.ctrl + (size_t)j * 3 * N + uv;

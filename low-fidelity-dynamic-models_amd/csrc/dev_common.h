@@ -19,6 +19,11 @@ constexpr double kHalfSqrt2 = 0.70710678118654757;              // sqrt(2.) / 2.
 // finite-difference checks run it.
 #define LFDM_HD __host__ __device__ __forceinline__
 
+// Keeps the compiler from carrying values read from memory across this point in registers.  Step
+// loops that keep per-vehicle constants in LDS put it at the top of every step, so each step reads
+// the constants where it uses them instead of holding them in registers for the whole loop.
+#define LFDM_RELOAD_MEMORY() asm volatile("" ::: "memory")
+
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 LFDM_HD double sgn(double v) { return (double)((0.0 < v) - (v < 0.0)); }

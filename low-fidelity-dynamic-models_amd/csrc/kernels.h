@@ -47,13 +47,13 @@ hipError_t launch_d24(const Launch& L, int block, hipStream_t stream);
 // controls ctrl[nsteps][3][n_user]; writes xy[nsteps][2][n_user] (position after each step) and
 // the carried state every K steps into ckpt[ceil(nsteps/K)][32][S].  Backward, one group of
 // segments [seg_lo, seg_hi) at a time, last group first: the recompute launch reruns the group's
-// segments from their checkpoints into tape[seg_hi - seg_lo][K][56][S] (carried state and tire
-// outputs of every step), the reverse launch walks them backwards.  From xy_bar (layout of xy) the
+// segments from their checkpoints into tape[seg_hi - seg_lo][S / 64][K][38][64] (the carried state and
+// tire outputs the step adjoint reads, every step, blocked by wavefront), the reverse launch walks them backwards.  From xy_bar (layout of xy) the
 // last group's reverse writes ctrl_bar (layout of ctrl), state_bar[75][n_user] (adjoint of the
 // initial record) and param_bar[7][n_user] (m, jz, jx, dfy0Pn, dfy0P2n, torque scale,
 // maxBrakeTorque); between groups the running adjoints live in carry[52][S].
 constexpr int kAdjCkptRow = 32;  // doubles per vehicle in a checkpoint
-constexpr int kAdjTapeRow = 56;  // doubles per vehicle and step in the tape
+constexpr int kAdjTapeRow = 38;  // doubles per vehicle and step in the tape
 constexpr int kAdjCarry = 52;    // doubles per vehicle in carry
 
 struct AdjLaunch {
@@ -73,7 +73,8 @@ struct AdjLaunch {
     double* carry;
     double dt;
     int nsteps, K, S, n_user;
-    int seg_lo, seg_hi;
+    int seg_lo, seg_hi;  // backward: the group of this launch
+    int fwd_tape_lo;     // forward: segments from this one on also go to the tape (block 0 onwards)
 };
 
 // waves_per_simd 1 or 2: register budget of the forward kernel (1 when the fleet fills at most one

@@ -145,6 +145,8 @@ struct lfdm_solver {
     double* d_carry = nullptr;
     size_t carry_cap = 0;
     int adj_nsteps = 0, adj_K = 0;
+    int adj_G = 1;                 // segments per backward group, fixed by the forward
+    bool tape_holds_last = false;  // the tape holds the last group, written by the forward
     size_t adj_tape_budget = size_t(16) << 30;  // bytes the backward's tape may take
     int simds = 0;  // SIMDs of the device (4 per CU), queried once
     const double* adj_ctrl = nullptr;
@@ -314,6 +316,7 @@ int drop_device_image(lfdm_solver* h) {
     dfree(h->d_wave_hmax);
     h->built = false;
     h->adj_nsteps = 0;  // a pending adjoint refers to the old slots
+    h->tape_holds_last = false;
     return LFDM_OK;
 }
 
@@ -1165,8 +1168,15 @@ int lfdm_adjoint_forward(lfdm_solver* h, int nsteps, int checkpoint_every, const
     h->adj_nsteps = nsteps;
     h->adj_K = K;
     h->adj_ctrl = dctrl;
+    h->tape_holds_last = false;
+    // the forward also tapes the group the backward walks first
+    const int G = adj_group(h, (int)nseg);
+    if (G < 0) return G;
+    h->adj_G = G;
+    if (int rc = ensure_buffer(h->d_tape, h->tape_cap, (size_t)G * K * kAdjTapeRow * (size_t)h->S)) return rc;
     AdjLaunch L = adj_launch(h);
     L.xy = dxy;
+    L.fwd_tape_lo = (int)nseg - G;
     // a fleet of at most one wavefront per SIMD gets the whole register file per lane
     const int waves = (size_t)h->S / kWave <= (size_t)simds ? 1 : 2;
     HIPCHK(hipEventRecord(h->ev0, h->stream));
@@ -1174,6 +1184,7 @@ int lfdm_adjoint_forward(lfdm_solver* h, int nsteps, int checkpoint_every, const
     if (e != hipSuccess) return fail(LFDM_E_HIP, std::string("adjoint forward launch failed: ") + hipGetErrorString(e));
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     h->timed = true;
+    h->tape_holds_last = true;
     return LFDM_OK;
 }
 
@@ -1189,9 +1200,7 @@ int lfdm_adjoint_backward(lfdm_solver* h, const double* dxy_bar, double* dctrl_b
     if (!dxy_bar || !dctrl_bar || !dstate_bar || !dparam_bar) return fail(LFDM_E_ARG, "null adjoint buffer");
     if (h->adj_nsteps <= 0) return fail(LFDM_E_STATE, "adjoint backward before adjoint forward");
     const int nseg = (h->adj_nsteps + h->adj_K - 1) / h->adj_K;
-    const int G = adj_group(h, nseg);
-    if (G < 0) return G;
-    if (int rc = ensure_buffer(h->d_tape, h->tape_cap, (size_t)G * h->adj_K * kAdjTapeRow * (size_t)h->S)) return rc;
+    const int G = h->adj_G;
     if (int rc = ensure_buffer(h->d_carry, h->carry_cap, (size_t)kAdjCarry * (size_t)h->S)) return rc;
     AdjLaunch L = adj_launch(h);
     L.xy_bar = dxy_bar;
@@ -1202,11 +1211,13 @@ int lfdm_adjoint_backward(lfdm_solver* h, const double* dxy_bar, double* dctrl_b
     for (int hi = nseg; hi > 0; hi -= G) {
         L.seg_hi = hi;
         L.seg_lo = std::max(hi - G, 0);
-        hipError_t e = launch_adj_recompute(L, h->stream);
+        hipError_t e = hipSuccess;
+        if (!(hi == nseg && h->tape_holds_last)) e = launch_adj_recompute(L, h->stream);
         if (e == hipSuccess) e = launch_adj_reverse(L, h->stream);
         if (e != hipSuccess)
             return fail(LFDM_E_HIP, std::string("adjoint backward launch failed: ") + hipGetErrorString(e));
     }
+    if (nseg > G) h->tape_holds_last = false;  // later groups overwrote it
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     h->timed = true;
     return LFDM_OK;

@@ -128,6 +128,7 @@ int main(int argc, char** argv) {
     L.K = K;
     L.S = S;
     L.n_user = n;
+    L.fwd_tape_lo = nseg - G;  // the forward tapes the last group, as the library does
     for (int s = 0; s < S; ++s) adj_forward_lane(L, s, P);
     std::vector<double> J(n, 0.0);
     v = 0;
@@ -148,8 +149,11 @@ int main(int argc, char** argv) {
     for (int hi = nseg; hi > 0; hi -= G) {
         L.seg_hi = hi;
         L.seg_lo = std::max(hi - G, 0);
-        for (int q = 0; q < hi - L.seg_lo; ++q)
-            for (int s = 0; s < S; ++s) adj_recompute_lane(L, q, s, P);
+        for (int q = 0; q < (hi == nseg ? 0 : hi - L.seg_lo); ++q)  // the last group is on the tape
+            for (int s = 0; s < S; ++s) {
+                K18 k;
+                adj_recompute_lane(L, q, s, P, k);
+            }
         for (int s = 0; s < S; ++s) {
             K18 k;
             K18b kb;
