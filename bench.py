@@ -225,7 +225,7 @@ def mixed_fleet(n_total=2_000_000, split=(0.4, 0.4, 0.2), tmin=2.0, tmax=20.0, j
             "value": work / wall, "unit": "vehicle-steps/s", "graph_ms": float(min(ms)), "wall_s": wall,
             "autotune": {"seconds": tune_s, "trial_steps": trial_steps,
                          "buckets": [{"dof": d_, "vehicles": int(len(h)), "block": c[0], "steps_per_launch": c[1],
-                                      "layout": "by_horizon" if c[2] else "grouped"}
+                                      "waves_per_simd": c[2], "layout": "by_horizon" if c[3] else "grouped"}
                                      for d_, h, c in zip((11, 18, 24), horizons, cfg)]},
             "lane_efficiency": {"by_horizon": [lane_efficiency(h, True) for h in horizons],
                                 "grouped": [lane_efficiency(h, False) for h in horizons]},

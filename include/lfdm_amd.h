@@ -205,12 +205,15 @@ int lfdm_fleet_add(lfdm_fleet* f, lfdm_solver* h);
 /* Stream for the fleet's runs from now on (hipStream_t; NULL is the default stream).  A fleet whose
  * caller never chose one uses a non-blocking stream of its own. */
 int lfdm_fleet_set_stream(lfdm_fleet* f, void* stream);
-/* Launch configuration of a bucket: threads per block (64..256, a multiple of 64) and steps per launch. */
-int lfdm_fleet_set_config(lfdm_fleet* f, int bucket, int block, int steps_per_launch);
-int lfdm_fleet_get_config(lfdm_fleet* f, int bucket, int* block, int* steps_per_launch, int* layout);
+/* Launch configuration of a bucket: threads per block (64..256, a multiple of 64), steps per launch
+ * and the kernel's waves per SIMD (0: one when the bucket fills at most one wavefront per SIMD, else
+ * the model's default; 1 or 2 for the 11/18-DOF kernels, 1 for the 24-DOF one). */
+int lfdm_fleet_set_config(lfdm_fleet* f, int bucket, int block, int steps_per_launch, int waves_per_simd);
+int lfdm_fleet_get_config(lfdm_fleet* f, int bucket, int* block, int* steps_per_launch, int* waves_per_simd,
+                          int* layout);
 /* Picks every bucket's layout (from the lane-steps each layout would run over nsteps steps with the
- * bucket's horizons) and its block size and steps per launch (timed trials of min(trial_steps,
- * nsteps) steps from the current records, which are restored afterwards). */
+ * bucket's horizons) and its block size, steps per launch and waves per SIMD (timed trials of
+ * min(trial_steps, nsteps) steps from the current records, which are restored afterwards). */
 int lfdm_fleet_autotune(lfdm_fleet* f, int nsteps, int trial_steps);
 int lfdm_fleet_run(lfdm_fleet* f, int nsteps);
 int lfdm_fleet_synchronize(lfdm_fleet* f);

@@ -71,7 +71,7 @@ LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, doubl
     double fx[2], fy[2], My[2], xt[2], rStat[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-        const double d = i == 0 ? delta : u.delta_rear;
+        const double d = i == 0 ? sd / cd : u.tan_delta_rear;
         if (fz[i] <= 0) {
             if (!NR) { s.xe[i] = 0; s.ye[i] = 0; }
             s.omega[i] = 0; xt[i] = 0; rStat[i] = t.r0; fx[i] = 0; fy[i] = 0; My[i] = 0;

@@ -209,7 +209,7 @@ struct TireBars {
 };
 
 // Adjoint of tire_tmeasy (dof18.cpp:380-539) for a tire on the ground (fz > 0).
-LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, double qy2, double delta, double fz,
+LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, double qy2, double tan_delta, double fz,
                              double vsx, double vsy, double xe, double ye, double omega, TireBars& B) {
     // ---- forward recompute ----
     const double xt = fz / t.kt;
@@ -227,10 +227,8 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     const double rw = r_eff * aom;
     const double vta = rw + 0.01;
     const double sx = -vsx2 / vta;
-    const double at = atan2(vsy, vta);
-    const double alpha = at - delta;
-    const double ta = tan(alpha);
-    const double sy = -ta;
+    const double sy = lateral_slip(vsy, vta, tan_delta);
+    const double ta = -sy;  // tan of the slip angle
     const bool clip = fz > t.pnmax;
     const double fzc = clip ? t.pnmax : fz;
     Curve c;
@@ -241,7 +239,7 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     const double hsxn = c.sxm / ssum + rx / rsum;
     const double hsyn = c.sym / ssum + ry / rsum;
     const double sxn = sx / hsxn, syn = sy / hsyn;
-    const double sc = hypot(sxn, syn);
+    const double sc = hyp(sxn, syn);
     double ca, sa;
     if (sc > 0) {
         ca = sxn / sc;
@@ -255,7 +253,7 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     const double E = c.sxm * ca / hsxn, F = c.sym * sa / hsyn;
     const double G = c.fxs * ca, H = c.fys * sa;
     const double I = c.sxs * ca / hsxn, J = c.sys * sa / hsyn;
-    const double df0 = hypot(A, Bq), fm = hypot(C, D), sm = hypot(E, F), fs = hypot(G, H), ss = hypot(I, J);
+    const double df0 = hyp(A, Bq), fm = hyp(C, D), sm = hyp(E, F), fs = hyp(G, H), ss = hyp(I, J);
     double f;
     const double fos = tmxy_fos(sc, df0, sm, fm, ss, fs, f);
     const double ssv = (vta <= 0.0) ? 0.0 : 1.0;
@@ -372,7 +370,7 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     B.qy1 += dfy0b * fzn;
     B.qy2 -= dfy0b * fzn * fzn;
     if (!clip) fzb += fznb * u.inv_pn;
-    // sy = -tan(alpha), alpha = atan2(vsy, vta) - delta
+    // sy = -tan(alpha), alpha = atan2(vsy, vta) - delta (evaluated as lateral_slip)
     const double alphab = -syb * (1.0 + ta * ta);
     B.delta -= alphab;
     const double ar2 = alphab / (vsy * vsy + vta * vta);
@@ -615,7 +613,7 @@ LFDM_HD void step18_adj(const PSet& P, const Derived& u, const K18& k, const S18
             yedot[i] = to->yedot[i];
         } else {
             double xt;
-            tire_tmeasy(t, u, k, i < 2 ? delta : u.delta_rear, fz[i], vsx[i], vsy[i], xe[i], ye[i], xedot[i],
+            tire_tmeasy(t, u, k, i < 2 ? sd / cd : u.tan_delta_rear, fz[i], vsx[i], vsy[i], xe[i], ye[i], xedot[i],
                         yedot[i], om[i], fx[i], fy[i], My[i], xt, rStat[i]);
         }
     }
@@ -801,7 +799,7 @@ LFDM_HD void step18_adj(const PSet& P, const Derived& u, const K18& k, const S18
         tb.My = pb.My[i];
         tb.rStat = pb.rStat[i];
         tb.fz = tb.vsx = tb.vsy = tb.delta = tb.xe = tb.ye = tb.omega = tb.qy1 = tb.qy2 = 0.0;
-        tire_tmeasy_adj(t, u, k.qy1, k.qy2, i < 2 ? delta : u.delta_rear, fz[i], vsx[i], vsy[i], s0.xe[i],
+        tire_tmeasy_adj(t, u, k.qy1, k.qy2, i < 2 ? sd / cd : u.tan_delta_rear, fz[i], vsx[i], vsy[i], s0.xe[i],
                         s0.ye[i], s0.omega[i], tb);
         sb.xe[i] += xeb[i] + tb.xe;
         sb.ye[i] += yeb[i] + tb.ye;

@@ -69,7 +69,7 @@ LFDM_HD void k18_init(const PSet& P, double m, double jz, double jx, double dfy0
 // Tire RHS, TMeasy (dof18.cpp:380-539): vertical load from the load transfer, static radius from the
 // tire compression, then the force core.  Off the ground the tire resets and keeps its deflection
 // rates (dof18.cpp:399-409).
-LFDM_HD void tire_tmeasy(const TireP& t, const Derived& u, const K18& k, double delta, double fz,
+LFDM_HD void tire_tmeasy(const TireP& t, const Derived& u, const K18& k, double tan_delta, double fz,
                                             double vsx, double vsy, double& xe, double& ye, double& xedot,
                                             double& yedot, double& omega, double& fx, double& fy, double& My,
                                             double& xt, double& rStat) {
@@ -79,11 +79,11 @@ LFDM_HD void tire_tmeasy(const TireP& t, const Derived& u, const K18& k, double 
     }
     xt = fz / t.kt;
     rStat = t.r0 - xt;
-    tmeasy_core(t, u, k.qy1, k.qy2, delta, fz, rStat, vsx, vsy, xe, ye, omega, xedot, yedot, fx, fy, My);
+    tmeasy_core(t, u, k.qy1, k.qy2, tan_delta, fz, rStat, vsx, vsy, xe, ye, omega, xedot, yedot, fx, fy, My);
 }
 
 // Tire RHS, TMeasyNr (dof18.cpp:541-673).
-LFDM_HD void tire_tmeasy_nr(const TireP& t, const Derived& u, const K18& k, double delta, double fz,
+LFDM_HD void tire_tmeasy_nr(const TireP& t, const Derived& u, const K18& k, double tan_delta, double fz,
                                                double vsx, double vsy, double& omega, double& fx, double& fy,
                                                double& My, double& xt, double& rStat) {
     if (fz <= 0) {
@@ -92,7 +92,7 @@ LFDM_HD void tire_tmeasy_nr(const TireP& t, const Derived& u, const K18& k, doub
     }
     xt = fz / t.kt;
     rStat = t.r0 - xt;
-    tmeasy_nr_core(t, u, k.qy1, k.qy2, delta, fz, rStat, vsx, vsy, omega, fx, fy, My);
+    tmeasy_nr_core(t, u, k.qy1, k.qy2, tan_delta, fz, rStat, vsx, vsy, omega, fx, fy, My);
 }
 
 // One half-implicit step (rhsFun order, dof18_halfImplicit.cpp:618-673; update :245-318).
@@ -138,7 +138,7 @@ LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, doubl
     double fx[4], fy[4], My[4], xt[4], rStat[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const double d = i < 2 ? delta : u.delta_rear;
+        const double d = i < 2 ? sd / cd : u.tan_delta_rear;
         if (!NR)
             tire_tmeasy(t, u, k, d, fz[i], vsx[i], vsy[i], s.xe[i], s.ye[i], s.xedot[i], s.yedot[i], s.omega[i],
                         fx[i], fy[i], My[i], xt[i], rStat[i]);

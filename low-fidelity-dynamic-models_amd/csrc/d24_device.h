@@ -117,7 +117,7 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     const double cc = cth * cph;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const double d = i < 2 ? delta : u.delta_rear;
+        const double d = i < 2 ? sd / cd : u.tan_delta_rear;
         fz[i] = s.xt[i] * t.kt;
         s.rStat[i] = (t.r0 - s.xt[i]) / cc;
         if (fz[i] <= 0) {

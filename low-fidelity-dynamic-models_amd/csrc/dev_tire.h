@@ -76,10 +76,27 @@ LFDM_HD void curve_at(const TireP& t, const Derived& u, double qy1, double qy2, 
     c.sys = t.sysPn + u.lsys * fzn1;
 }
 
+// The two places where the tire cores evaluate the reference's functions in another form (same
+// function, other rounding; the difference is of the size of device libm's last-bit differences
+// from glibc, far inside the 1e-9 parity bar):
+//  - lateral slip sy = -tan(atan2(vsy, vta) - delta) (dof18.cpp:430-431) as the tangent difference
+//    -(r - tan delta) / (1 + r tan delta), r = vsy / vta (vta > 0, so tan(atan2(vsy, vta)) = r):
+//    one division instead of an atan2 and a tan per tire; tan delta comes once per step from the
+//    step's sin/cos of the steering angle;
+//  - hypot(a, b) as sqrt(a a + b b): the tire quantities stay hundreds of orders of magnitude away
+//    from the overflow and underflow that hypot's scaling guards against.
+LFDM_HD double lateral_slip(double vsy, double vta, double tan_delta) {
+    const double r = vsy / vta;
+    return -((r - tan_delta) / (1. + r * tan_delta));
+}
+
+LFDM_HD double hyp(double a, double b) { return sqrt(a * a + b * b); }
+
 // TMeasy force core (dof18.cpp:412-539; the same code is dof11.cpp:335-472 and dof24.cpp:851-987)
 // for a tire on the ground (fz > 0) with static radius rStat.  Returns the deflection rates and
-// the forces; vsx, vsy are the contact-patch velocities of the transform.
-LFDM_HD void tmeasy_core(const TireP& t, const Derived& u, double qy1, double qy2, double delta,
+// the forces; vsx, vsy are the contact-patch velocities of the transform, tan_delta the tangent of
+// the tire's steering angle.
+LFDM_HD void tmeasy_core(const TireP& t, const Derived& u, double qy1, double qy2, double tan_delta,
                                             double fz, double rStat, double vsx, double vsy, double xe, double ye,
                                             double omega, double& xedot, double& yedot, double& fx, double& fy,
                                             double& My) {
@@ -93,8 +110,7 @@ LFDM_HD void tmeasy_core(const TireP& t, const Derived& u, double qy1, double qy
     const double rw = r_eff * fabs(omega);
     const double vta = rw + 0.01;
     const double sx = -vsx / vta;
-    const double alpha = atan2(vsy, vta) - delta;
-    const double sy = -tan(alpha);
+    const double sy = lateral_slip(vsy, vta, tan_delta);
     if (fz > t.pnmax) fz = t.pnmax;
     Curve c;
     curve_at(t, u, qy1, qy2, fz, c);
@@ -104,7 +120,7 @@ LFDM_HD void tmeasy_core(const TireP& t, const Derived& u, double qy1, double qy
     const double hsyn = c.sym / ssum + ry / rsum;
     const double sxn = sx / hsxn;
     const double syn = sy / hsyn;
-    const double sc = hypot(sxn, syn);
+    const double sc = hyp(sxn, syn);
     double ca, sa;
     if (sc > 0) {
         ca = sxn / sc;
@@ -113,11 +129,11 @@ LFDM_HD void tmeasy_core(const TireP& t, const Derived& u, double qy1, double qy
         ca = kHalfSqrt2;
         sa = kHalfSqrt2;
     }
-    const double df0 = hypot(c.dfx0 * ca * hsxn, c.dfy0 * sa * hsyn);
-    const double fm = hypot(c.fxm * ca, c.fym * sa);
-    const double sm = hypot(c.sxm * ca / hsxn, c.sym * sa / hsyn);
-    const double fs = hypot(c.fxs * ca, c.fys * sa);
-    const double ss = hypot(c.sxs * ca / hsxn, c.sys * sa / hsyn);
+    const double df0 = hyp(c.dfx0 * ca * hsxn, c.dfy0 * sa * hsyn);
+    const double fm = hyp(c.fxm * ca, c.fym * sa);
+    const double sm = hyp(c.sxm * ca / hsxn, c.sym * sa / hsyn);
+    const double fs = hyp(c.fxs * ca, c.fys * sa);
+    const double ss = hyp(c.sxs * ca / hsxn, c.sys * sa / hsyn);
     double f;
     const double fos = tmxy_fos(sc, df0, sm, fm, ss, fs, f);
     // -sineStep(vta, 0, 0, 0, 1) * rr * fz * rStat * sgn(omega); the step is 0 for vta <= 0, else 1
@@ -145,7 +161,7 @@ LFDM_HD void tmeasy_core(const TireP& t, const Derived& u, double qy1, double qy
 // TMeasyNr force core (dof18.cpp:571-673 with the Coulomb blend of :290-306; the same code is
 // dof11.cpp:503-606 and dof24.cpp:1014-1121) for a tire on the ground with static radius rStat.
 LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double qy2,
-                                               double delta, double fz, double rStat, double vsx, double vsy,
+                                               double tan_delta, double fz, double rStat, double vsx, double vsy,
                                                double omega, double& fx, double& fy, double& My) {
     const double vsx_in = vsx;
     double rdynco;
@@ -158,19 +174,19 @@ LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double
     const double vta = r_eff * fabs(omega) + 0.01;
     double Fx0 = tanh(-2.0 * vsx / t.vcoulomb) * fz * t.mu;
     double Fy0 = tanh(-2.0 * vsy / t.vcoulomb) * fz * t.mu;
-    if (hypot(Fx0, Fy0) > fz * t.mu) {
-        const double fsc = fz * t.mu / hypot(Fx0, Fy0);
+    const double F0 = hyp(Fx0, Fy0);
+    if (F0 > fz * t.mu) {
+        const double fsc = fz * t.mu / F0;
         Fx0 = Fx0 * fsc;
         Fy0 = Fy0 * fsc;
     }
     const double sx = -vsx / vta;
-    const double alpha = atan2(vsy, vta) - delta;
-    const double sy = -tan(alpha);
+    const double sy = lateral_slip(vsy, vta, tan_delta);
     if (fz > t.pnmax) fz = t.pnmax;
     Curve c;
     curve_at(t, u, qy1, qy2, fz, c);
     const double frblend = sine_step(fabs(vsx_in), t.frblend_begin, 0., t.frblend_end, 1.);
-    const double sc = hypot(sx, sy);
+    const double sc = hyp(sx, sy);
     double ca, sa;
     if (sc > 0) {
         ca = sx / sc;
@@ -179,11 +195,11 @@ LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double
         ca = kHalfSqrt2;
         sa = kHalfSqrt2;
     }
-    const double df0 = hypot(c.dfx0 * ca, c.dfy0 * sa);
-    const double fm = t.mu * hypot(c.fxm * ca, c.fym * sa);
-    const double sm = t.mu * hypot(c.sxm * ca, c.sym * sa);
-    const double fs = t.mu * hypot(c.fxs * ca, c.fys * sa);
-    const double ss = t.mu * hypot(c.sxs * ca, c.sys * sa);
+    const double df0 = hyp(c.dfx0 * ca, c.dfy0 * sa);
+    const double fm = t.mu * hyp(c.fxm * ca, c.fym * sa);
+    const double sm = t.mu * hyp(c.sxm * ca, c.sym * sa);
+    const double fs = t.mu * hyp(c.fxs * ca, c.fys * sa);
+    const double ss = t.mu * hyp(c.sxs * ca, c.sys * sa);
     double f;
     tmxy_fos(sc, df0, sm, fm, ss, fs, f);
     double Fx, Fy;

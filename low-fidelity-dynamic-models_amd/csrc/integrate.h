@@ -326,21 +326,15 @@ struct M24 {
     }
 };
 
-// Occupancy target of the integration kernels (waves per SIMD), per model.  The 11- and 18-DOF
-// steps need more registers than two waves per SIMD leave (the compiler spills some to scratch),
-// yet two resident waves hide more FP64 latency than the spills cost (18-DOF, 1M vehicles: 5.0e9
-// versus 3.7e9 vehicle-steps/s at one wave; 11-DOF 9.4e9 versus 5.8e9).  The 24-DOF step would
-// spill over a kilobyte per lane at two waves and runs 2.6x faster at one (3.3e9 versus 1.3e9).
-// LFDM_WAVES_PER_EU overrides every model (0 leaves it to the compiler).
-#ifdef LFDM_WAVES_PER_EU
-#define LFDM_WAVES(M) LFDM_WAVES_PER_EU
-#else
-#define LFDM_WAVES(M) (M::kWaves)
-#endif
-
-template <class M, bool NR, int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LFDM_WAVES(M), LFDM_WAVES(M)))) void integrate(
-    Launch L) {
+// Occupancy of the integration kernels (waves per SIMD).  Each model has a default, M::kWaves: the
+// 11- and 18-DOF steps need more registers than two waves per SIMD leave (the compiler spills some
+// to scratch), yet on large fleets two resident waves hide more FP64 latency than the spills cost;
+// the 24-DOF step would spill over a kilobyte per lane at two waves and runs 2.6x faster at one.
+// A fleet that fills at most one wavefront per SIMD gains nothing from the second wave's registers
+// budget, so the 11- and 18-DOF kernels also exist at one wave per SIMD (no spills); Launch::waves
+// picks (0: the model's default).
+template <class M, bool NR, int MODE, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) void integrate(Launch L) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int wave = uniform(slot >> 6);
     if (wave * kWave >= L.S) return;
@@ -409,22 +403,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LFDM_WAVES(
     }
 }
 
-template <class M, bool NR>
+template <class M, bool NR, int W>
 hipError_t launch_mode(const Launch& L, int block, hipStream_t stream) {
     const int grid = (L.S + block - 1) / block;
     switch (L.mode) {
-        case kCtrlTable: hipLaunchKernelGGL((integrate<M, NR, kCtrlTable>), grid, block, 0, stream, L); break;
-        case kCtrlConst: hipLaunchKernelGGL((integrate<M, NR, kCtrlConst>), grid, block, 0, stream, L); break;
-        case kCtrlKnots: hipLaunchKernelGGL((integrate<M, NR, kCtrlKnots>), grid, block, 0, stream, L); break;
-        case kCtrlSteps: hipLaunchKernelGGL((integrate<M, NR, kCtrlSteps>), grid, block, 0, stream, L); break;
+        case kCtrlTable: hipLaunchKernelGGL((integrate<M, NR, kCtrlTable, W>), grid, block, 0, stream, L); break;
+        case kCtrlConst: hipLaunchKernelGGL((integrate<M, NR, kCtrlConst, W>), grid, block, 0, stream, L); break;
+        case kCtrlKnots: hipLaunchKernelGGL((integrate<M, NR, kCtrlKnots, W>), grid, block, 0, stream, L); break;
+        case kCtrlSteps: hipLaunchKernelGGL((integrate<M, NR, kCtrlSteps, W>), grid, block, 0, stream, L); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+template <class M, int W>
+hipError_t launch_tire(const Launch& L, int block, hipStream_t stream) {
+    return L.nr ? launch_mode<M, true, W>(L, block, stream) : launch_mode<M, false, W>(L, block, stream);
+}
+
 template <class M>
 hipError_t launch_model(const Launch& L, int block, hipStream_t stream) {
-    return L.nr ? launch_mode<M, true>(L, block, stream) : launch_mode<M, false>(L, block, stream);
+    if constexpr (M::kWaves == 2) {
+        if (L.waves != 1) return launch_tire<M, 2>(L, block, stream);
+    }
+    return launch_tire<M, 1>(L, block, stream);
 }
 
 }  // namespace kern

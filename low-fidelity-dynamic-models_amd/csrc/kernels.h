@@ -35,6 +35,7 @@ struct Launch {
     const int* horizon;
     const int* wave_hmax;
     int step_base;
+    int waves;  // waves per SIMD of the kernel: 1, 2, or 0 for the model's default
 };
 
 // Integration launches of the 11-, 18- and 24-DOF models (one kernel per tire model and control

@@ -41,7 +41,8 @@ struct Derived {
     double two_ab, E2c, Uc, A1, A1sq, jxz2, kro, bro, inv_jw, inv_ci;
     double qx1, qx2, qfxm1, qfxm2, qfym1, qfym2, qfxs1, qfxs2, qfys1, qfys2;  // InterpQ pairs (2w1 - w2/2, w1 - w2/2)
     double lsxm, lsym, lsxs, lsys, lrd;                                       // InterpL slopes (w2 - w1)
-    double delta_rear;  // steering angle of the rear tires (steering input 0)
+    double delta_rear;      // steering angle of the rear tires (steering input 0)
+    double tan_delta_rear;  // its tangent
     // reciprocals and quotients only the 18-DOF step adjoint reads
     double inv_pn, inv_kt, inv_two_ab, krof_cf, kror_cr, brof_cf, bror_cr;
 };

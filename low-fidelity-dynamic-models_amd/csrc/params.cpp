@@ -397,6 +397,7 @@ void derive18(PSet& p) {
     u.lsys = t.sysP2n - t.sysPn;
     u.lrd = t.rdyncoP2n - t.rdyncoPn;
     u.delta_rear = p.nonLinearSteer ? host_map_y(p.steer, 0.0) : 0.0 * p.maxSteer;
+    u.tan_delta_rear = std::tan(u.delta_rear);
     u.inv_pn = 1. / t.pn;
     u.inv_kt = 1. / t.kt;
     u.inv_two_ab = 1. / u.two_ab;

@@ -160,7 +160,8 @@ struct lfdm_solver {
 struct lfdm_fleet {
     struct Cfg {
         int block = 256;
-        int spl = 0;  // steps per launch, 0 = the whole run in one launch
+        int spl = 0;    // steps per launch, 0 = the whole run in one launch
+        int waves = 0;  // waves per SIMD of the kernel, 0 = the solver's rule
     };
     std::vector<lfdm_solver*> buckets;
     std::vector<Cfg> cfg;
@@ -402,6 +403,16 @@ std::vector<int> output_steps(int nsteps, double step, double dtout) {
     return r;
 }
 
+// SIMDs of the current device (4 per CU), queried once per handle.
+int simd_count(lfdm_solver* h) {
+    if (h->simds > 0) return h->simds;
+    int dev = 0, cus = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    h->simds = 4 * std::max(cus, 1);
+    return h->simds;
+}
+
 // Launch record of one integration launch of h (no outputs).  Zero-filled first, so records can be
 // compared bytewise.
 Launch make_launch(lfdm_solver* h, int mode, double t0, int nsteps, double cs = 0, double ct = 0, double cb = 0,
@@ -436,6 +447,9 @@ Launch make_launch(lfdm_solver* h, int mode, double t0, int nsteps, double cs = 
         L.wave_hmax = h->d_wave_hmax;
         L.step_base = (int)std::llround(t0 / h->step);
     }
+    // a fleet of at most one wavefront per SIMD runs the kernels built for one wave per SIMD
+    const int simds = simd_count(h);
+    L.waves = (simds > 0 && h->S / kWave <= simds) ? 1 : 0;
     return L;
 }
 
@@ -531,7 +545,7 @@ int check_controls(lfdm_solver* h, int nsteps) {
 
 // Launch records that integrate h nsteps steps from t = 0 (step 0) in launches of at most spl
 // steps, the time accumulated step by step as Solve does.  Ends with the final time in *t_end.
-std::vector<Launch> run_launches(lfdm_solver* h, int nsteps, int spl, double* t_end) {
+std::vector<Launch> run_launches(lfdm_solver* h, int nsteps, int spl, int waves, double* t_end) {
     const double* steps = h->ctrl_mode == kCtrlSteps ? (h->steps_ext ? h->steps_ext : h->d_steps) : nullptr;
     const size_t row = h->steps_shared ? 3 : 3 * (size_t)h->total;
     std::vector<Launch> r;
@@ -540,6 +554,7 @@ std::vector<Launch> run_launches(lfdm_solver* h, int nsteps, int spl, double* t_
         const int n = std::min(spl > 0 ? spl : nsteps, nsteps - done);
         Launch L = make_launch(h, h->ctrl_mode, t, n, 0, 0, 0, steps ? steps + (size_t)done * row : nullptr);
         if (L.horizon) L.step_base = done;
+        if (waves > 0) L.waves = waves;
         r.push_back(L);
         for (int k = 0; k < n; ++k) t += h->step;
         done += n;
@@ -1130,14 +1145,6 @@ AdjLaunch adj_launch(lfdm_solver* h) {
     return L;
 }
 
-int simd_count(lfdm_solver* h) {
-    if (h->simds > 0) return h->simds;
-    int dev = 0, cus = 0;
-    HIPCHK(hipGetDevice(&dev));
-    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    h->simds = 4 * std::max(cus, 1);
-    return h->simds;
-}
 
 // Segments per backward group: enough (segment, vehicle) lanes for about four wavefronts per SIMD
 // in the recompute launch, within the handle's tape budget (at least one segment).
@@ -1263,6 +1270,9 @@ int fleet_check(lfdm_fleet* f) {
     return LFDM_OK;
 }
 
+// Occupancies the model's integration kernels are built for (24-DOF: one wave per SIMD only).
+int max_waves(int dof) { return dof == 24 ? 1 : 2; }
+
 void fleet_drop_graph(lfdm_fleet* f) {
     if (f->exec) (void)hipGraphExecDestroy(f->exec);
     if (f->graph) (void)hipGraphDestroy(f->graph);
@@ -1371,21 +1381,28 @@ int lfdm_fleet_set_stream(lfdm_fleet* f, void* stream) {
     return LFDM_OK;
 }
 
-int lfdm_fleet_set_config(lfdm_fleet* f, int bucket, int block, int steps_per_launch) {
+int lfdm_fleet_set_config(lfdm_fleet* f, int bucket, int block, int steps_per_launch, int waves_per_simd) {
     if (int rc = fleet_check(f)) return rc;
     if (bucket < 0 || bucket >= (int)f->buckets.size()) return fail(LFDM_E_ARG, "bad bucket index");
     if (block < 64 || block > 256 || block % kWave) return fail(LFDM_E_ARG, "block size must be 64, 128, 192 or 256");
     if (steps_per_launch < 0) return fail(LFDM_E_ARG, "steps per launch must be >= 0");
+    if (waves_per_simd < 0 || waves_per_simd > max_waves(f->buckets[bucket]->dof))
+        return fail(LFDM_E_ARG, "waves per SIMD must be 0 (automatic) or 1" +
+                                    std::string(max_waves(f->buckets[bucket]->dof) > 1 ? " or 2" : "") +
+                                    " for the " + std::to_string(f->buckets[bucket]->dof) + "-DOF kernel");
     f->cfg[bucket].block = block;
     f->cfg[bucket].spl = steps_per_launch;
+    f->cfg[bucket].waves = waves_per_simd;
     return LFDM_OK;
 }
 
-int lfdm_fleet_get_config(lfdm_fleet* f, int bucket, int* block, int* steps_per_launch, int* layout) {
+int lfdm_fleet_get_config(lfdm_fleet* f, int bucket, int* block, int* steps_per_launch, int* waves_per_simd,
+                          int* layout) {
     if (int rc = fleet_check(f)) return rc;
     if (bucket < 0 || bucket >= (int)f->buckets.size()) return fail(LFDM_E_ARG, "bad bucket index");
     if (block) *block = f->cfg[bucket].block;
     if (steps_per_launch) *steps_per_launch = f->cfg[bucket].spl;
+    if (waves_per_simd) *waves_per_simd = f->cfg[bucket].waves;
     if (layout) *layout = f->buckets[bucket]->layout;
     return LFDM_OK;
 }
@@ -1397,7 +1414,7 @@ int lfdm_fleet_run(lfdm_fleet* f, int nsteps) {
     std::vector<double> t_end(f->buckets.size());
     std::vector<char> sig;
     for (size_t b = 0; b < f->buckets.size(); ++b) {
-        ls[b] = run_launches(f->buckets[b], nsteps, f->cfg[b].spl, &t_end[b]);
+        ls[b] = run_launches(f->buckets[b], nsteps, f->cfg[b].spl, f->cfg[b].waves, &t_end[b]);
         const char* blk = reinterpret_cast<const char*>(&f->cfg[b].block);
         sig.insert(sig.end(), blk, blk + sizeof(int));
         for (const Launch& L : ls[b]) {
@@ -1467,26 +1484,27 @@ int lfdm_fleet_autotune(lfdm_fleet* f, int nsteps, int trial_steps) {
         int rc = LFDM_OK;
         const int blocks[3] = {64, 128, 256};
         const int parts[2] = {1, 4};
-        for (int bi = 0; bi < 3 && rc == LFDM_OK && e == hipSuccess; ++bi)
-            for (int pi = 0; pi < 2 && rc == LFDM_OK && e == hipSuccess; ++pi) {
-                const std::vector<Launch> ls = run_launches(h, T, (T + parts[pi] - 1) / parts[pi], nullptr);
-                float ms = 0;
-                for (int rep = 0; rep < 2 && rc == LFDM_OK && e == hipSuccess; ++rep) {
-                    e = hipMemcpyAsync(h->d_state, saved, n * sizeof(double), hipMemcpyDeviceToDevice, f->stream);
-                    if (e == hipSuccess) e = hipEventRecord(f->ev0, f->stream);
-                    if (e == hipSuccess) rc = launch_list(h, ls, blocks[bi], f->stream);
-                    if (e == hipSuccess && rc == LFDM_OK) e = hipEventRecord(f->ev1, f->stream);
-                    if (e == hipSuccess && rc == LFDM_OK) e = hipEventSynchronize(f->ev1);
-                    float x = 0;
-                    if (e == hipSuccess && rc == LFDM_OK) e = hipEventElapsedTime(&x, f->ev0, f->ev1);
-                    if (rep == 0 || x < ms) ms = x;
-                }
-                if (e == hipSuccess && rc == LFDM_OK && (best == 0 || ms < best)) {
-                    best = ms;
-                    pick.block = blocks[bi];
-                    pick.spl = parts[pi] == 1 ? 0 : (nsteps + parts[pi] - 1) / parts[pi];
-                }
+        for (int trial = 0; trial < max_waves(h->dof) * 6 && rc == LFDM_OK && e == hipSuccess; ++trial) {
+            const int wv = 1 + trial / 6, bi = (trial / 2) % 3, pi = trial % 2;
+            const std::vector<Launch> ls = run_launches(h, T, (T + parts[pi] - 1) / parts[pi], wv, nullptr);
+            float ms = 0;
+            for (int rep = 0; rep < 2 && rc == LFDM_OK && e == hipSuccess; ++rep) {
+                e = hipMemcpyAsync(h->d_state, saved, n * sizeof(double), hipMemcpyDeviceToDevice, f->stream);
+                if (e == hipSuccess) e = hipEventRecord(f->ev0, f->stream);
+                if (e == hipSuccess) rc = launch_list(h, ls, blocks[bi], f->stream);
+                if (e == hipSuccess && rc == LFDM_OK) e = hipEventRecord(f->ev1, f->stream);
+                if (e == hipSuccess && rc == LFDM_OK) e = hipEventSynchronize(f->ev1);
+                float x = 0;
+                if (e == hipSuccess && rc == LFDM_OK) e = hipEventElapsedTime(&x, f->ev0, f->ev1);
+                if (rep == 0 || x < ms) ms = x;
             }
+            if (e == hipSuccess && rc == LFDM_OK && (best == 0 || ms < best)) {
+                best = ms;
+                pick.block = blocks[bi];
+                pick.spl = parts[pi] == 1 ? 0 : (nsteps + parts[pi] - 1) / parts[pi];
+                pick.waves = wv;
+            }
+        }
         if (e == hipSuccess) e = hipMemcpyAsync(h->d_state, saved, n * sizeof(double), hipMemcpyDeviceToDevice, f->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
         (void)hipFree(saved);

@@ -107,8 +107,9 @@ SIGNATURES = {
     "lfdm_fleet_destroy": [_p],
     "lfdm_fleet_add": [_p, _p],
     "lfdm_fleet_set_stream": [_p, _p],
-    "lfdm_fleet_set_config": [_p, _i, _i, _i],
-    "lfdm_fleet_get_config": [_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],
+    "lfdm_fleet_set_config": [_p, _i, _i, _i, _i],
+    "lfdm_fleet_get_config": [_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                              ctypes.POINTER(_i)],
     "lfdm_fleet_autotune": [_p, _i, _i],
     "lfdm_fleet_run": [_p, _i],
     "lfdm_fleet_synchronize": [_p],

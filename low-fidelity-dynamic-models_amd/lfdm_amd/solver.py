@@ -321,14 +321,15 @@ class Fleet:
     def set_stream(self, stream_ptr):
         check(lib.lfdm_fleet_set_stream(self._f, ctypes.c_void_p(stream_ptr)))
 
-    def set_config(self, bucket, block, steps_per_launch=0):
-        check(lib.lfdm_fleet_set_config(self._f, bucket, block, steps_per_launch))
+    def set_config(self, bucket, block, steps_per_launch=0, waves_per_simd=0):
+        check(lib.lfdm_fleet_set_config(self._f, bucket, block, steps_per_launch, waves_per_simd))
 
     def get_config(self, bucket):
-        """(block, steps_per_launch, layout) of a bucket."""
-        b, s, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        check(lib.lfdm_fleet_get_config(self._f, bucket, ctypes.byref(b), ctypes.byref(s), ctypes.byref(l)))
-        return b.value, s.value, l.value
+        """(block, steps_per_launch, waves_per_simd, layout) of a bucket."""
+        b, s, w, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib.lfdm_fleet_get_config(self._f, bucket, ctypes.byref(b), ctypes.byref(s), ctypes.byref(w),
+                                        ctypes.byref(l)))
+        return b.value, s.value, w.value, l.value
 
     def autotune(self, nsteps, trial_steps=1000):
         check(lib.lfdm_fleet_autotune(self._f, nsteps, trial_steps))

@@ -48,12 +48,12 @@ def test_fleet_arguments():
     with pytest.raises(LfdmError) as e:  # a handle is one bucket
         f.add(s)
     assert e.value.name == "LFDM_E_ARG"
-    for bad in ((1, 256, 0), (0, 96, 0), (0, 512, 0), (0, 256, -1)):
+    for bad in ((1, 256, 0), (0, 96, 0), (0, 512, 0), (0, 256, -1), (0, 256, 0, 3)):
         with pytest.raises(LfdmError) as e:
             f.set_config(*bad)
         assert e.value.name == "LFDM_E_ARG"
-    f.set_config(0, 128, 500)
-    assert f.get_config(0) == (128, 500, LAYOUT_GROUPED)
+    f.set_config(0, 128, 500, 1)
+    assert f.get_config(0) == (128, 500, 1, LAYOUT_GROUPED)
     with pytest.raises(LfdmError) as e:
         f.autotune(100, trial_steps=0)
     assert e.value.name == "LFDM_E_ARG"

@@ -150,8 +150,8 @@ def mixed(with_horizons=True, T=400):
 
 def test_fleet_graph_matches_eager_runs():
     """Mixed 11/18/24-DOF fleet with horizons: three graph replays (records reset between them, eager
-    work in between) each bit-identical to eager launches; a new launch configuration recaptures
-    the graph and still gives the same bits."""
+    work in between) each bit-identical to eager launches; a new launch configuration (block size,
+    steps per launch, one or two waves per SIMD) recaptures the graph and still gives the same bits."""
     say("[batching] fleet graph vs eager")
     T = 400
     a, b = mixed(T=T), mixed(T=T)
@@ -169,7 +169,8 @@ def test_fleet_graph_matches_eager_runs():
         say(f"[batching] replay {rep}: {f.last_ms():.2f} ms")
         for (dof, _, _), g_, r_ in zip(MIX, got, ref):
             np.testing.assert_array_equal(g_, r_, err_msg=f"replay {rep}, {dof}-DOF")
-    f.set_config(1, 64, 150)
+    f.set_config(0, 256, 0, 1)
+    f.set_config(1, 64, 150, 2)
     f.set_config(2, 128, 0)
     for s, i0 in zip(a, init):
         s.set_states(i0)
@@ -193,11 +194,12 @@ def test_autotune_keeps_records_and_results():
     before = [s.get_states() for s in a]
     f = Fleet(a)
     cfg = f.autotune(T, trial_steps=100)
-    say(f"[batching] tuned (block, steps per launch, layout): {cfg}")
+    say(f"[batching] tuned (block, steps per launch, waves per SIMD, layout): {cfg}")
     for s, r_ in zip(a, before):
         np.testing.assert_array_equal(s.get_states(), r_)
-    assert all(c[2] == LAYOUT_BY_HORIZON for c in cfg)
+    assert all(c[3] == LAYOUT_BY_HORIZON for c in cfg)
     assert all(c[0] in (64, 128, 256) for c in cfg)
+    assert cfg[2][2] == 1 and all(c[2] in (1, 2) for c in cfg)
     f.run(T)
     for (dof, _, _), s, r_ in zip(MIX, a, ref):
         np.testing.assert_array_equal(s.get_states(), r_, err_msg=f"{dof}-DOF")
