@@ -147,7 +147,8 @@ def mixed_fleet(n_total=2_000_000, split=(0.4, 0.4, 0.2), tmin=2.0, tmax=20.0, j
     autotuner picks layout / block size / steps per launch, then each timed run is one graph
     launch over the longest horizon (records reset between runs, outside the timed region).
     Throughput counts only the steps each vehicle takes (sum of its horizon steps).  For
-    comparison: the same fleet in Construct order with one eager launch per model in turn."""
+    comparison: the same fleet in vehicle order, one eager launch per model, each on its handle's
+    own stream."""
     import lfdm_amd
     from lfdm_amd import LAYOUT_GROUPED, Fleet, SolverHalfImplicitGPU, TMeasy
     from lfdm_amd import workloads as W
@@ -205,7 +206,7 @@ def mixed_fleet(n_total=2_000_000, split=(0.4, 0.4, 0.2), tmin=2.0, tmax=20.0, j
         walls.append(time.perf_counter() - t0)
         ms.append(f.last_ms())
     wall = min(walls)
-    # reference-style comparison: Construct order, one eager launch per model after the other
+    # comparison: vehicle order, one eager launch per model on each handle's own stream
     for s in buckets:
         s.set_layout(LAYOUT_GROUPED)
     reset()
