@@ -77,7 +77,7 @@ LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, doubl
             s.omega[i] = 0; xt[i] = 0; rStat[i] = t.r0; fx[i] = 0; fy[i] = 0; My[i] = 0;
             continue;
         }
-        xt[i] = fz[i] / t.kt;
+        xt[i] = fz[i] * u.inv_kt;
         rStat[i] = t.r0 - xt[i];
         if (!NR)
             tmeasy_core(t, u, k.qy1, k.qy2, d, fz[i], rStat[i], vsx[i], vsy[i], s.xe[i], s.ye[i], s.omega[i],

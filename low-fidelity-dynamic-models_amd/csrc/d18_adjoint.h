@@ -211,66 +211,26 @@ struct TireBars {
 // Adjoint of tire_tmeasy (dof18.cpp:380-539) for a tire on the ground (fz > 0).
 LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, double qy2, double tan_delta, double fz,
                              double vsx, double vsy, double xe, double ye, double omega, TireBars& B) {
-    // ---- forward recompute ----
-    const double xt = fz / t.kt;
-    const double rStat = t.r0 - xt;
-    double rdynco, drd = 0.0;
-    if (fz <= t.fzRdynco) {
-        rdynco = t.rdyncoPn + u.lrd * (fz / t.pn - 1.);
-        drd = u.lrd / t.pn;
-    } else {
-        rdynco = t.rdyncoCrit;
-    }
-    const double r_eff = rdynco * t.r0 + (1. - rdynco) * rStat;
-    const double vsx2 = vsx - omega * r_eff;
-    const double aom = fabs(omega), som = sgn(omega);
-    const double rw = r_eff * aom;
-    const double vta = rw + 0.01;
-    const double sx = -vsx2 / vta;
-    const double sy = lateral_slip(vsy, vta, tan_delta);
-    const double ta = -sy;  // tan of the slip angle
-    const bool clip = fz > t.pnmax;
-    const double fzc = clip ? t.pnmax : fz;
-    Curve c;
-    curve_at(t, u, qy1, qy2, fzc, c);
-    const double fzn = fzc / t.pn;
-    const double rx = c.fxm / c.dfx0, ry = c.fym / c.dfy0;
-    const double ssum = c.sxm + c.sym, rsum = rx + ry;
-    const double hsxn = c.sxm / ssum + rx / rsum;
-    const double hsyn = c.sym / ssum + ry / rsum;
-    const double sxn = sx / hsxn, syn = sy / hsyn;
-    const double sc = hyp(sxn, syn);
-    double ca, sa;
-    if (sc > 0) {
-        ca = sxn / sc;
-        sa = syn / sc;
-    } else {
-        ca = kHalfSqrt2;
-        sa = kHalfSqrt2;
-    }
-    const double A = c.dfx0 * ca * hsxn, Bq = c.dfy0 * sa * hsyn;
-    const double C = c.fxm * ca, D = c.fym * sa;
-    const double E = c.sxm * ca / hsxn, F = c.sym * sa / hsyn;
-    const double G = c.fxs * ca, H = c.fys * sa;
-    const double I = c.sxs * ca / hsxn, J = c.sys * sa / hsyn;
-    const double df0 = hyp(A, Bq), fm = hyp(C, D), sm = hyp(E, F), fs = hyp(G, H), ss = hyp(I, J);
-    double f;
-    const double fos = tmxy_fos(sc, df0, sm, fm, ss, fs, f);
-    const double ssv = (vta <= 0.0) ? 0.0 : 1.0;
-    const double vtxs = rw * hsxn + 0.01, vtys = rw * hsyn + 0.01;
-    const double vy = -sy * vta;
-    const double numx = -vtxs * t.cx * xe - fos * vsx2, denx = vtxs * t.dx + fos;
-    const double numy = -vtys * t.cy * ye - fos * vy, deny = vtys * t.dy + fos;
-    const double xedot = numx / denx, yedot = numy / deny;
-    const double fxdyn = t.dx * xedot + t.cx * xe;
-    const double fydyn = t.dy * yedot + t.cy * ye;
-    const double fxr = xe * t.cx + xedot * t.dx, fyr = ye * t.cy + yedot * t.dy;
-    const double fxstr = clampd(fxr, -t.fxmP2n, t.fxmP2n), fystr = clampd(fyr, -t.fymP2n, t.fymP2n);
-    const double avx = fabs(vsx2), avy = fabs(vy);
-    const double wxw = sine_step_weight(avx), wyw = sine_step_weight(avy);
+    // ---- forward values (the step's own evaluation) ----
+    const double rStat = t.r0 - fz * u.inv_kt;
+    TireEval w;
+    tmeasy_eval(t, u, qy1, qy2, tan_delta, fz, rStat, vsx, vsy, xe, ye, omega, w);
+    const double rdynco = w.rdynco, drd = w.drd, r_eff = w.r_eff, vsx2 = w.vsx2, aom = w.aom, som = w.som;
+    const double rw = w.rw, vta = w.vta, sx = w.sx, sy = w.sy, ta = -sy;  // ta: tan of the slip angle
+    const bool clip = w.clip;
+    const double fzc = w.fzc;
+    const Curve& c = w.c;
+    const double fzn = w.fzn, rx = w.rx, ry = w.ry, hsxn = w.hsxn, hsyn = w.hsyn;
+    const double sxn = w.sxn, syn = w.syn, sc = w.sc, ca = w.ca, sa = w.sa;
+    const double A = w.A, Bq = w.B, C = w.C, D = w.D, E = w.E, F = w.F, G = w.G, H = w.H, I = w.I, J = w.J;
+    const double df0 = w.df0, fm = w.fm, sm = w.sm, fs = w.fs, ss = w.ss, fos = w.fos, ssv = w.ssv;
+    const double vtxs = w.vtxs, vtys = w.vtys, vy = w.vy, numx = w.numx, denx = w.denx, numy = w.numy;
+    const double deny = w.deny, xedot = w.xedot, yedot = w.yedot, fxdyn = w.fxdyn, fydyn = w.fydyn;
+    const double fxr = w.fxr, fyr = w.fyr, fxstr = w.fxstr, fystr = w.fystr, avx = w.avx, avy = w.avy;
+    const double wxw = w.wxw, wyw = w.wyw;
 
     // ---- reverse ----
-    const double rhsxn = 1.0 / hsxn, rhsyn = 1.0 / hsyn, rvta = 1.0 / vta;
+    const double rhsxn = w.rhsxn, rhsyn = w.rhsyn, rvta = 1.0 / vta;
     double vsx2b = 0, vyb = 0, xedotb = B.xedot, yedotb = B.yedot, xeb = 0, yeb = 0;
     // fx = wxw fxstr + (1 - wxw) fxdyn
     const double wxwb = B.fx * (fxstr - fxdyn), fxstrb = B.fx * wxw, fxdynb = B.fx * (1. - wxw);
@@ -300,9 +260,9 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     // vtxs = rw hsxn + 0.01, vtys = rw hsyn + 0.01
     double rwb = vtxsb * hsxn + vtysb * hsyn;
     double hsxnb = vtxsb * rw, hsynb = vtysb * rw;
-    // My = -ssv rr fz rStat sgn(omega)
-    double fzb = B.My * (-ssv * t.rr * rStat * som);
-    double rStatb = B.rStat + B.My * (-ssv * t.rr * fz * som);
+    // My = -ssv rr fzc rStat sgn(omega), fzc = min(fz, pnmax)
+    double fzb = clip ? 0.0 : B.My * (-ssv * t.rr * rStat * som);
+    double rStatb = B.rStat + B.My * (-ssv * t.rr * fzc * som);
     // tmxy
     double scb = 0, df0b = 0, smb = 0, fmb = 0, ssb = 0, fsb = 0;
     tmxy_adj(sc, df0, sm, fm, ss, fs, 0.0, fosb, scb, df0b, smb, fmb, ssb, fsb);
@@ -346,7 +306,7 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     syb += synbr;
     hsynb -= synbr * syn;
     // hsxn = sxm/ssum + rx/rsum, hsyn = sym/ssum + ry/rsum
-    const double rssum = 1.0 / ssum, rrsum = 1.0 / rsum;
+    const double rssum = w.rssum, rrsum = w.rrsum;
     sxmb += hsxnb * rssum;
     symb += hsynb * rssum;
     const double ssumb = -(hsxnb * c.sxm + hsynb * c.sym) * rssum * rssum;
@@ -370,7 +330,7 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     B.qy1 += dfy0b * fzn;
     B.qy2 -= dfy0b * fzn * fzn;
     if (!clip) fzb += fznb * u.inv_pn;
-    // sy = -tan(alpha), alpha = atan2(vsy, vta) - delta (evaluated as lateral_slip)
+    // sy = -tan(alpha), alpha = atan2(vsy, vta) - delta (evaluated as a tangent difference)
     const double alphab = -syb * (1.0 + ta * ta);
     B.delta -= alphab;
     const double ar2 = alphab / (vsy * vsy + vta * vta);

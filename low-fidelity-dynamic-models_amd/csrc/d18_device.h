@@ -77,7 +77,7 @@ LFDM_HD void tire_tmeasy(const TireP& t, const Derived& u, const K18& k, double 
         xe = 0; ye = 0; omega = 0; xt = 0; rStat = t.r0; fx = 0; fy = 0; My = 0;
         return;
     }
-    xt = fz / t.kt;
+    xt = fz * u.inv_kt;
     rStat = t.r0 - xt;
     tmeasy_core(t, u, k.qy1, k.qy2, tan_delta, fz, rStat, vsx, vsy, xe, ye, omega, xedot, yedot, fx, fy, My);
 }
@@ -90,7 +90,7 @@ LFDM_HD void tire_tmeasy_nr(const TireP& t, const Derived& u, const K18& k, doub
         omega = 0; xt = 0; rStat = t.r0; fx = 0; fy = 0; My = 0;
         return;
     }
-    xt = fz / t.kt;
+    xt = fz * u.inv_kt;
     rStat = t.r0 - xt;
     tmeasy_nr_core(t, u, k.qy1, k.qy2, tan_delta, fz, rStat, vsx, vsy, omega, fx, fy, My);
 }

@@ -25,9 +25,9 @@ LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, d
         } else if (s < sm) {
             const double p = df0loc * sm / fm - 2.0;
             const double sn = s / sm;
-            const double dn = 1.0 + (sn + p) * sn;
-            f = df0loc * sm * sn / dn;
-            fos = df0loc / dn;
+            const double rdn = 1.0 / (1.0 + (sn + p) * sn);
+            f = df0loc * sm * sn * rdn;
+            fos = df0loc * rdn;
         } else {
             const double q = fm / sm;
             const double a = q * q / (df0loc * sm);
@@ -60,7 +60,7 @@ struct Curve {
 // qy1, qy2: InterpQ coefficients of the (per-vehicle) initial lateral slope dfy0.
 LFDM_HD void curve_at(const TireP& t, const Derived& u, double qy1, double qy2, double fz,
                                          Curve& c) {
-    const double fzn = fz / t.pn;
+    const double fzn = fz * u.inv_pn;
     const double fzn1 = fzn - 1.;
     // InterpQ(fz, w1, w2, pn) = (fz/pn) * (2 w1 - 0.5 w2 - (w1 - 0.5 w2) (fz/pn)), utils.h:98-100
     c.dfx0 = fzn * (u.qx1 - u.qx2 * fzn);
@@ -76,86 +76,135 @@ LFDM_HD void curve_at(const TireP& t, const Derived& u, double qy1, double qy2, 
     c.sys = t.sysPn + u.lsys * fzn1;
 }
 
-// The two places where the tire cores evaluate the reference's functions in another form (same
-// function, other rounding; the difference is of the size of device libm's last-bit differences
-// from glibc, far inside the 1e-9 parity bar):
+// Where the tire cores evaluate the reference's functions in another form (same function, other
+// rounding; the differences are of the size of device libm's last-bit differences from glibc, far
+// inside the 1e-9 parity bar):
 //  - lateral slip sy = -tan(atan2(vsy, vta) - delta) (dof18.cpp:430-431) as the tangent difference
 //    -(r - tan delta) / (1 + r tan delta), r = vsy / vta (vta > 0, so tan(atan2(vsy, vta)) = r):
-//    one division instead of an atan2 and a tan per tire; tan delta comes once per step from the
-//    step's sin/cos of the steering angle;
-//  - hypot(a, b) as sqrt(a a + b b): the tire quantities stay hundreds of orders of magnitude away
-//    from the overflow and underflow that hypot's scaling guards against.
-LFDM_HD double lateral_slip(double vsy, double vta, double tan_delta) {
-    const double r = vsy / vta;
-    return -((r - tan_delta) / (1. + r * tan_delta));
-}
-
+//    no atan2 and no tan per tire; tan delta comes once per step from the step's sin/cos of the
+//    steering angle;
+//  - hypot(a, b) as sqrt(a a + b b) (hyp): the tire quantities stay hundreds of orders of magnitude
+//    away from the overflow and underflow that hypot's scaling guards against;
+//  - divisions by a shared denominator (vta, hsxn, hsyn, sc, sxm + sym, rx + ry, pn, kt) as
+//    multiplications by its reciprocal.
 LFDM_HD double hyp(double a, double b) { return sqrt(a * a + b * b); }
 
-// TMeasy force core (dof18.cpp:412-539; the same code is dof11.cpp:335-472 and dof24.cpp:851-987)
-// for a tire on the ground (fz > 0) with static radius rStat.  Returns the deflection rates and
-// the forces; vsx, vsy are the contact-patch velocities of the transform, tan_delta the tangent of
-// the tire's steering angle.
-LFDM_HD void tmeasy_core(const TireP& t, const Derived& u, double qy1, double qy2, double tan_delta,
-                                            double fz, double rStat, double vsx, double vsy, double xe, double ye,
-                                            double omega, double& xedot, double& yedot, double& fx, double& fy,
-                                            double& My) {
-    double rdynco;
-    if (fz <= t.fzRdynco)
-        rdynco = t.rdyncoPn + u.lrd * (fz / t.pn - 1.);
-    else
-        rdynco = t.rdyncoCrit;
-    const double r_eff = rdynco * t.r0 + (1. - rdynco) * rStat;
-    vsx = vsx - (omega * r_eff);
-    const double rw = r_eff * fabs(omega);
-    const double vta = rw + 0.01;
-    const double sx = -vsx / vta;
-    const double sy = lateral_slip(vsy, vta, tan_delta);
-    if (fz > t.pnmax) fz = t.pnmax;
+// Intermediate values of the TMeasy force core (dof18.cpp:412-539; the same code is
+// dof11.cpp:335-472 and dof24.cpp:851-987) for a tire on the ground (fz > 0) with static radius
+// rStat: the step reads the outputs (deflection rates, forces, rolling moment), the 18-DOF adjoint
+// the rest, so both see the same values.
+struct TireEval {
+    double rdynco, drd, r_eff, vsx2, aom, som, rw, vta, sx, sy;
+    bool clip;
+    double fzc;
     Curve c;
-    curve_at(t, u, qy1, qy2, fz, c);
-    const double rx = c.fxm / c.dfx0, ry = c.fym / c.dfy0;
-    const double ssum = c.sxm + c.sym, rsum = rx + ry;
-    const double hsxn = c.sxm / ssum + rx / rsum;
-    const double hsyn = c.sym / ssum + ry / rsum;
-    const double sxn = sx / hsxn;
-    const double syn = sy / hsyn;
-    const double sc = hyp(sxn, syn);
-    double ca, sa;
-    if (sc > 0) {
-        ca = sxn / sc;
-        sa = syn / sc;
+    double fzn, rx, ry, ssum, rsum, rssum, rrsum, hsxn, hsyn, rhsxn, rhsyn, sxn, syn, sc, ca, sa;
+    double A, B, C, D, E, F, G, H, I, J, df0, fm, sm, fs, ss, f, fos;
+    double ssv, vtxs, vtys, vy, numx, denx, numy, deny, xedot, yedot;
+    double fxdyn, fydyn, fxr, fyr, fxstr, fystr, avx, avy, wxw, wyw, fx, fy, My;
+};
+
+// vsx, vsy are the contact-patch velocities of the transform, tan_delta the tangent of the tire's
+// steering angle.
+LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy2, double tan_delta, double fz,
+                         double rStat, double vsx, double vsy, double xe, double ye, double omega, TireEval& w) {
+    if (fz <= t.fzRdynco) {
+        w.rdynco = t.rdyncoPn + u.lrd * (fz * u.inv_pn - 1.);
+        w.drd = u.lrd * u.inv_pn;
     } else {
-        ca = kHalfSqrt2;
-        sa = kHalfSqrt2;
+        w.rdynco = t.rdyncoCrit;
+        w.drd = 0.0;
     }
-    const double df0 = hyp(c.dfx0 * ca * hsxn, c.dfy0 * sa * hsyn);
-    const double fm = hyp(c.fxm * ca, c.fym * sa);
-    const double sm = hyp(c.sxm * ca / hsxn, c.sym * sa / hsyn);
-    const double fs = hyp(c.fxs * ca, c.fys * sa);
-    const double ss = hyp(c.sxs * ca / hsxn, c.sys * sa / hsyn);
-    double f;
-    const double fos = tmxy_fos(sc, df0, sm, fm, ss, fs, f);
-    // -sineStep(vta, 0, 0, 0, 1) * rr * fz * rStat * sgn(omega); the step is 0 for vta <= 0, else 1
-    const double ssv = (vta <= 0.0) ? 0.0 : 1.0;
-    My = -ssv * t.rr * fz * rStat * sgn(omega);
-    const double vtxs = rw * hsxn + 0.01;
-    const double vtys = rw * hsyn + 0.01;
-    const double vy = -sy * vta;
-    const double numx = -vtxs * t.cx * xe - fos * vsx;
-    const double denx = vtxs * t.dx + fos;
-    const double numy = -vtys * t.cy * ye - fos * vy;
-    const double deny = vtys * t.dy + fos;
-    xedot = numx / denx;
-    yedot = numy / deny;
-    const double fxdyn = t.dx * numx / denx + t.cx * xe;
-    const double fydyn = t.dy * yedot + (t.cy * ye);
-    const double fxstr = clampd(xe * t.cx + xedot * t.dx, -t.fxmP2n, t.fxmP2n);
-    const double fystr = clampd(ye * t.cy + yedot * t.dy, -t.fymP2n, t.fymP2n);
-    const double wx = sine_step_weight(fabs(vsx));
-    const double wy = sine_step_weight(fabs(vy));
-    fx = wx * fxstr + (1. - wx) * fxdyn;
-    fy = wy * fystr + (1. - wy) * fydyn;
+    w.r_eff = w.rdynco * t.r0 + (1. - w.rdynco) * rStat;
+    w.vsx2 = vsx - (omega * w.r_eff);
+    w.aom = fabs(omega);
+    w.som = sgn(omega);
+    w.rw = w.r_eff * w.aom;
+    w.vta = w.rw + 0.01;
+    const double rvta = 1. / w.vta;
+    w.sx = -w.vsx2 * rvta;
+    const double r = vsy * rvta;  // tan(atan2(vsy, vta))
+    w.sy = -((r - tan_delta) / (1. + r * tan_delta));
+    w.clip = fz > t.pnmax;
+    w.fzc = w.clip ? t.pnmax : fz;
+    const Curve& c = w.c;
+    curve_at(t, u, qy1, qy2, w.fzc, w.c);
+    w.fzn = w.fzc * u.inv_pn;
+    w.rx = c.fxm / c.dfx0;
+    w.ry = c.fym / c.dfy0;
+    w.ssum = c.sxm + c.sym;
+    w.rsum = w.rx + w.ry;
+    w.rssum = 1. / w.ssum;
+    w.rrsum = 1. / w.rsum;
+    w.hsxn = c.sxm * w.rssum + w.rx * w.rrsum;
+    w.hsyn = c.sym * w.rssum + w.ry * w.rrsum;
+    w.rhsxn = 1. / w.hsxn;
+    w.rhsyn = 1. / w.hsyn;
+    w.sxn = w.sx * w.rhsxn;
+    w.syn = w.sy * w.rhsyn;
+    w.sc = hyp(w.sxn, w.syn);
+    if (w.sc > 0) {
+        const double rsc = 1. / w.sc;
+        w.ca = w.sxn * rsc;
+        w.sa = w.syn * rsc;
+    } else {
+        w.ca = kHalfSqrt2;
+        w.sa = kHalfSqrt2;
+    }
+    w.A = c.dfx0 * w.ca * w.hsxn;
+    w.B = c.dfy0 * w.sa * w.hsyn;
+    w.C = c.fxm * w.ca;
+    w.D = c.fym * w.sa;
+    w.E = c.sxm * w.ca * w.rhsxn;
+    w.F = c.sym * w.sa * w.rhsyn;
+    w.G = c.fxs * w.ca;
+    w.H = c.fys * w.sa;
+    w.I = c.sxs * w.ca * w.rhsxn;
+    w.J = c.sys * w.sa * w.rhsyn;
+    w.df0 = hyp(w.A, w.B);
+    w.fm = hyp(w.C, w.D);
+    w.sm = hyp(w.E, w.F);
+    w.fs = hyp(w.G, w.H);
+    w.ss = hyp(w.I, w.J);
+    w.fos = tmxy_fos(w.sc, w.df0, w.sm, w.fm, w.ss, w.fs, w.f);
+    // -sineStep(vta, 0, 0, 0, 1) * rr * fz * rStat * sgn(omega) with the limited fz; the step is 0
+    // for vta <= 0, else 1
+    w.ssv = (w.vta <= 0.0) ? 0.0 : 1.0;
+    w.My = -w.ssv * t.rr * w.fzc * rStat * w.som;
+    w.vtxs = w.rw * w.hsxn + 0.01;
+    w.vtys = w.rw * w.hsyn + 0.01;
+    w.vy = -w.sy * w.vta;
+    w.numx = -w.vtxs * t.cx * xe - w.fos * w.vsx2;
+    w.denx = w.vtxs * t.dx + w.fos;
+    w.numy = -w.vtys * t.cy * ye - w.fos * w.vy;
+    w.deny = w.vtys * t.dy + w.fos;
+    w.xedot = w.numx / w.denx;
+    w.yedot = w.numy / w.deny;
+    w.fxdyn = t.dx * w.xedot + t.cx * xe;
+    w.fydyn = t.dy * w.yedot + (t.cy * ye);
+    w.fxr = xe * t.cx + w.xedot * t.dx;
+    w.fyr = ye * t.cy + w.yedot * t.dy;
+    w.fxstr = clampd(w.fxr, -t.fxmP2n, t.fxmP2n);
+    w.fystr = clampd(w.fyr, -t.fymP2n, t.fymP2n);
+    w.avx = fabs(w.vsx2);
+    w.avy = fabs(w.vy);
+    w.wxw = sine_step_weight(w.avx);
+    w.wyw = sine_step_weight(w.avy);
+    w.fx = w.wxw * w.fxstr + (1. - w.wxw) * w.fxdyn;
+    w.fy = w.wyw * w.fystr + (1. - w.wyw) * w.fydyn;
+}
+
+// TMeasy force core: deflection rates, forces and rolling moment of a tire on the ground.
+LFDM_HD void tmeasy_core(const TireP& t, const Derived& u, double qy1, double qy2, double tan_delta, double fz,
+                         double rStat, double vsx, double vsy, double xe, double ye, double omega, double& xedot,
+                         double& yedot, double& fx, double& fy, double& My) {
+    TireEval w;
+    tmeasy_eval(t, u, qy1, qy2, tan_delta, fz, rStat, vsx, vsy, xe, ye, omega, w);
+    xedot = w.xedot;
+    yedot = w.yedot;
+    fx = w.fx;
+    fy = w.fy;
+    My = w.My;
 }
 
 // TMeasyNr force core (dof18.cpp:571-673 with the Coulomb blend of :290-306; the same code is
@@ -166,7 +215,7 @@ LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double
     const double vsx_in = vsx;
     double rdynco;
     if (fz <= t.fzRdynco)
-        rdynco = t.rdyncoPn + u.lrd * (fz / t.pn - 1.);
+        rdynco = t.rdyncoPn + u.lrd * (fz * u.inv_pn - 1.);
     else
         rdynco = t.rdyncoCrit;
     const double r_eff = rdynco * t.r0 + (1. - rdynco) * rStat;
@@ -180,17 +229,20 @@ LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double
         Fx0 = Fx0 * fsc;
         Fy0 = Fy0 * fsc;
     }
-    const double sx = -vsx / vta;
-    const double sy = lateral_slip(vsy, vta, tan_delta);
+    const double rvta = 1. / vta;
+    const double sx = -vsx * rvta;
+    const double r = vsy * rvta;  // tan(atan2(vsy, vta))
+    const double sy = -((r - tan_delta) / (1. + r * tan_delta));
     if (fz > t.pnmax) fz = t.pnmax;
     Curve c;
     curve_at(t, u, qy1, qy2, fz, c);
     const double frblend = sine_step(fabs(vsx_in), t.frblend_begin, 0., t.frblend_end, 1.);
     const double sc = hyp(sx, sy);
     double ca, sa;
+    const double rsc = sc > 0 ? 1. / sc : 0.;
     if (sc > 0) {
-        ca = sx / sc;
-        sa = sy / sc;
+        ca = sx * rsc;
+        sa = sy * rsc;
     } else {
         ca = kHalfSqrt2;
         sa = kHalfSqrt2;
@@ -204,8 +256,8 @@ LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double
     tmxy_fos(sc, df0, sm, fm, ss, fs, f);
     double Fx, Fy;
     if (sc > 0.) {
-        Fx = f * sx / sc;
-        Fy = f * sy / sc;
+        Fx = f * sx * rsc;
+        Fy = f * sy * rsc;
     } else {
         Fx = 0.;
         Fy = 0.;
