@@ -115,11 +115,12 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     // computeTireRHS (dof24.cpp:827-1121): load from the compression, radius from the attitude
     double fx[4], fy[4], fz[4], My[4];
     const double cc = cth * cph;
+    const double rcc = 1. / cc;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const double d = i < 2 ? sd / cd : u.tan_delta_rear;
         fz[i] = s.xt[i] * t.kt;
-        s.rStat[i] = (t.r0 - s.xt[i]) / cc;
+        s.rStat[i] = (t.r0 - s.xt[i]) * rcc;
         if (fz[i] <= 0) {
             if (!NR) { s.xe[i] = 0; s.ye[i] = 0; }
             s.omega[i] = 0; s.xt[i] = 0; s.rStat[i] = t.r0; fz[i] = 0; fx[i] = 0; fy[i] = 0; My[i] = 0;
@@ -181,12 +182,12 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
         fzs[i] = s.xs[i] * P.ks + s.dxs[i] * P.bs;
     }
     s.fzd[1] = (fyg[1] * s.rStat[1] + fys[1] * ls[1] + fyg[0] * s.rStat[0] + fys[0] * ls[0] -
-                (fys[1] + fys[0]) * P.hrcf) /
-               cf;
+                (fys[1] + fys[0]) * P.hrcf) *
+               u.inv_cf;
     s.fzd[0] = -s.fzd[1];
     s.fzd[3] = (fyg[3] * s.rStat[3] + fys[3] * ls[3] + fyg[2] * s.rStat[2] + fys[2] * ls[2] -
-                (fys[3] + fys[2]) * P.hrcr) /
-               cr;
+                (fys[3] + fys[2]) * P.hrcr) *
+               u.inv_cr;
     s.fzd[2] = -s.fzd[3];
     double mx[4], my[4];
     mx[0] = fys[0] * P.hrcf;
@@ -229,9 +230,12 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     s.wx = s.wx + wxdot * dt;
     s.wy = s.wy + wydot * dt;
     s.wz = s.wz + wzdot * dt;
-    const double tth = tan(s.theta);
+    // Euler-angle rates at the step's (old) theta and phi: tan(theta) and 1/cos(theta) from the
+    // step's sine and cosine (same values, other rounding than tan() and two divisions)
+    const double rcth = 1. / cth;
+    const double tth = sth * rcth;
     const double dtheta = s.wy * cph - s.wz * sph;
-    const double dpsi = (s.wy * sph / cth) + (s.wz * cph / cth);
+    const double dpsi = (s.wy * sph * rcth) + (s.wz * cph * rcth);
     const double dphi = s.wx + s.wy * sph * tth + s.wz * cph * tth;
     s.theta = s.theta + dtheta * dt;
     s.psi = s.psi + dpsi * dt;

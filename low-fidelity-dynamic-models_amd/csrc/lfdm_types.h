@@ -43,8 +43,8 @@ struct Derived {
     double lsxm, lsym, lsxs, lsys, lrd;                                       // InterpL slopes (w2 - w1)
     double delta_rear;      // steering angle of the rear tires (steering input 0)
     double tan_delta_rear;  // its tangent
-    // reciprocals and quotients only the 18-DOF step adjoint reads
-    double inv_pn, inv_kt, inv_two_ab, krof_cf, kror_cr, brof_cf, bror_cr;
+    // reciprocals and quotients of parameters
+    double inv_pn, inv_kt, inv_two_ab, krof_cf, kror_cr, brof_cf, bror_cr, inv_cf, inv_cr;
 };
 
 // Vehicle parameters of one set (11-DOF dof11.h:496-620, 18-DOF dof18.h:489-799, 24-DOF

@@ -405,6 +405,8 @@ void derive18(PSet& p) {
     u.kror_cr = p.kror / p.cr;
     u.brof_cf = p.brof / p.cf;
     u.bror_cr = p.bror / p.cr;
+    u.inv_cf = 1. / p.cf;
+    u.inv_cr = 1. / p.cr;
 }
 
 }  // namespace
