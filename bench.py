@@ -11,9 +11,13 @@ steps) in one launch, with every input already resident in HBM.
 
 Prints one JSON line; `roofline` uses the algorithmic FP64 FLOPs per vehicle-step counted by the
 oracle's counting scalar on the same workload (+ - * / sqrt count 1, sin cos tan atan2 tanh
-count 20, hypot counts 4) and the device time of the integration launch from HIP events on the solver's stream.  `cpu_baseline`
-times the oracle (the CPU restatement, bit-identical to the reference solver) with OpenMP over
-vehicles on a bounded sample of the same workload.
+count 20, hypot counts 4) and the device time of the integration launch from HIP events on the
+solver's stream.  `cpu_baseline` times the oracle (the CPU restatement, bit-identical to the
+reference solver) with OpenMP over vehicles on a bounded sample of the same workload.  The default
+18-DOF line also carries `other_models` (the 24-DOF configuration, 500,000 vehicles over the same
+horizon, and the 11-DOF model on the headline fleet), `adjoint` (the differentiation form on 65,536
+vehicles x 5,000 steps, checkpoints every 100, with 20 MPC iterations) and `mixed_fleet` (the
+batching form on 2,000,000 vehicles of the three models with horizons U(2, 20) s).
 """
 import argparse
 import json
@@ -136,6 +140,47 @@ def adjoint_cost(flat, n=65536, T=5000, K=100, mpc_iterations=20):
                     "mean_loss_last": hist[-1] if hist else None},
             "note": "ratio = (adjoint forward + backward) / adjoint forward device time; central_fd_cost_ratio = "
                     "cost of central differences (2 plain forwards per input) / adjoint cost"}
+
+
+def hmmwv_params(dof):
+    import lfdm_amd
+    from lfdm_amd import workloads as W
+
+    d = os.path.join(W.DATA, f"{dof}dof", "HMMWV")
+    return lfdm_amd.params_from_json(dof, f"{d}/vehicle.json", f"{d}/tmeasy.json",
+                                     f"{d}/suspension.json" if dof == 24 else None)
+
+
+def knot_fleet_rate(dof, flat, n, H, steps, warmup, tpb=256):
+    """knot_fleet on n vehicles of the model, H solver steps per bench step (one launch), inputs
+    resident in HBM: (vehicle-steps/s over the timed steps, their wall time, mean device time of a
+    launch from HIP events on the solver's stream).  24-DOF vehicles start at U(0, 20) m/s."""
+    from lfdm_amd import SolverHalfImplicitGPU, TMeasy
+    from lfdm_amd import workloads as W
+
+    fl = W.knot_fleet(n, flat, seed=2, dof=dof)
+    s = SolverHalfImplicitGPU(dof, n)
+    s.construct_flat(flat, n, None, TMeasy)
+    s.set_overrides(fl["ovr"])
+    if dof == 24:
+        s.initialize_batch(W.initial_speed_records(n, flat, dof, seed=7))
+    else:
+        s.Initialize(s.zero_record(), n)
+    s.set_controls_knots(fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
+    s.SetThreadsPerBlock(tpb)
+    del fl
+    for _ in range(warmup):  # also uploads the fleet
+        s.advance(H)
+    s.synchronize()
+    launch_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.advance(H)
+        launch_ms.append(s.last_launch_ms())  # waits for this launch's end event
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    s.close()
+    return n * H * steps / wall, wall, float(np.mean(launch_ms))
 
 
 def mixed_fleet(n_total=2_000_000, split=(0.4, 0.4, 0.2), tmin=2.0, tmax=20.0, jitter=0.15, trial_steps=1000,
@@ -274,6 +319,7 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true", help="only the differentiation-form measurement")
     ap.add_argument("--mpc-iterations", type=int, default=20, help="MPC iterations of the adjoint measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the batching-form (mixed fleet) measurement")
+    ap.add_argument("--no-other-models", action="store_true", help="skip the 11- and 24-DOF lines of the default run")
     ap.add_argument("--mixed-only", action="store_true", help="only the batching-form measurement")
     ap.add_argument("--mixed-vehicles", type=int, default=2_000_000)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -295,49 +341,18 @@ def main():
                                    "communication between them, run with --gpus 1"}))
         sys.exit(2)
 
-    import lfdm_amd
-    from lfdm_amd import SolverHalfImplicitGPU, TMeasy
-    from lfdm_amd import workloads as W
-
     if args.mixed_only:
         print(json.dumps(mixed_fleet(args.mixed_vehicles)))
         return
 
     if args.adjoint_only:
-        d = os.path.join(W.DATA, "18dof", "HMMWV")
-        print(json.dumps(adjoint_cost(lfdm_amd.params_from_json(18, f"{d}/vehicle.json", f"{d}/tmeasy.json"),
-                                      mpc_iterations=args.mpc_iterations)))
+        print(json.dumps(adjoint_cost(hmmwv_params(18), mpc_iterations=args.mpc_iterations)))
         return
 
     n, H, dof = args.vehicles, args.horizon_steps, args.dof
-    d = os.path.join(W.DATA, f"{dof}dof", "HMMWV")
-    flat = lfdm_amd.params_from_json(dof, f"{d}/vehicle.json", f"{d}/tmeasy.json",
-                                     f"{d}/suspension.json" if dof == 24 else None)
-    fl = W.knot_fleet(n, flat, seed=2, dof=dof)
-    s = SolverHalfImplicitGPU(dof, n)
-    s.construct_flat(flat, n, None, TMeasy)
-    s.set_overrides(fl["ovr"])
-    if dof == 24:
-        s.initialize_batch(W.initial_speed_records(n, flat, dof, seed=7))
-    else:
-        s.Initialize(s.zero_record(), n)
-    s.set_controls_knots(fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
-    s.SetThreadsPerBlock(args.tpb)
-    del fl
-    # warmup (also uploads the fleet)
-    for _ in range(args.warmup):
-        s.advance(H)
-    s.synchronize()
-    launch_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s.advance(H)
-        launch_ms.append(s.last_launch_ms())  # waits for this launch's end event
-    s.synchronize()
-    wall = time.perf_counter() - t0
+    flat = hmmwv_params(dof)
+    value, wall, kernel_ms = knot_fleet_rate(dof, flat, n, H, args.steps, args.warmup, args.tpb)
     ms_per_step = 1e3 * wall / args.steps
-    value = n * H * args.steps / wall
-    kernel_ms = float(np.mean(launch_ms))
     fl_total, fl_basic = flops_per_step(dof, flat)
     achieved_tf = fl_total * n * H / (kernel_ms * 1e-3) / 1e12
     tps = measured_traffic_per_step() if dof == 18 else None
@@ -359,7 +374,16 @@ def main():
                "single_core": rate1, "cpu": cpu_model()}
     metric = ("18-DOF FP64 vehicle-steps/s at 1M vehicles; adjoint/forward cost ratio" if dof == 18 else
               f"{dof}-DOF FP64 vehicle-steps/s")
-    s.close()
+    other = None
+    if dof == 18 and not args.no_other_models:
+        # BASELINE's 24-DOF configuration (500,000 vehicles, 10 s) and the 11-DOF model on the headline fleet
+        other = {}
+        for d_, n_ in ((24, 500_000), (11, 1_000_000)):
+            v_, _, k_ = knot_fleet_rate(d_, hmmwv_params(d_), n_, H, 2, 1, args.tpb)
+            f_, _ = flops_per_step(d_, hmmwv_params(d_))
+            other[f"{d_}dof"] = {"value": v_, "unit": "vehicle-steps/s", "vehicles": n_, "horizon_steps": H,
+                                 "steps": 2, "kernel_ms": k_,
+                                 "fp64_frac": f_ * n_ * H / (k_ * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
     adj = None
     if dof == 18 and not args.no_adjoint:
         adj = adjoint_cost(flat, mpc_iterations=args.mpc_iterations)
@@ -376,7 +400,8 @@ def main():
                                    (", start at U(0,20) m/s" if dof == 24 else ""),
                        "vehicles": n, "horizon_steps": H, "dt": 1e-3, "threads_per_block": args.tpb},
             "roofline": roofline, "cpu_baseline": cpu, "adjoint": adj,
-            "adjoint_forward_cost_ratio": None if adj is None else adj["ratio"], "mixed_fleet": mixed}
+            "adjoint_forward_cost_ratio": None if adj is None else adj["ratio"], "mixed_fleet": mixed,
+            "other_models": other}
     print(json.dumps(line))
 
 
