@@ -527,10 +527,10 @@ LFDM_HD void step18_adj(const PSet& P, const Derived& u, const K18& k, const S18
     // ================= forward recompute =================
     const double lat = s0.vdot + s0.wz * s0.u;
     const double Z2f = k.Z2fc * lat;
-    const double Z3f = (P.krof * s0.phi + P.brof * s0.wx) / P.cf;
-    const double Z4 = (k.Z4n * (s0.udot - s0.wz * s0.v)) / u.two_ab;
+    const double Z3f = (P.krof * s0.phi + P.brof * s0.wx) * u.inv_cf;
+    const double Z4 = (k.Z4n * (s0.udot - s0.wz * s0.v)) * u.inv_two_ab;
     const double Z2r = k.Z2rc * lat;
-    const double Z3r = (P.kror * s0.phi + P.bror * s0.wx) / P.cr;
+    const double Z3r = (P.kror * s0.phi + P.bror * s0.wx) * u.inv_cr;
     double fzr[4], fz[4];
     fzr[0] = k.Z1f - Z2f - Z3f - Z4;
     fzr[1] = k.Z1f + Z2f + Z3f - Z4;
@@ -598,12 +598,12 @@ LFDM_HD void step18_adj(const PSet& P, const Derived& u, const K18& k, const S18
                       (gfx[3] - gfx[2]) * P.cr / 2 + u.E2c * s0.wz * s0.u;
     const double E3 = k.E3c * s0.phi - u.kro * s0.phi - u.bro * s0.wx + k.hm * s0.wz * s0.u;
     const double inner = (gfx[0] + gfx[1] + gfx[2] + gfx[3]) + u.Uc * (s0.wz * s0.wz) - k.two_hrc_m * s0.wz * s0.wx;
-    const double vdot = (E1 * u.jxz2 - k.A1A2 * E2 + A1 * E3 * jxz + A3 * E2 * jxz - A2 * E1 * jz - A3 * E3 * jz) /
-                        k.denom;
+    const double vdot = (E1 * u.jxz2 - k.A1A2 * E2 + A1 * E3 * jxz + A3 * E2 * jxz - A2 * E1 * jz - A3 * E3 * jz) *
+                        k.rdenom;
     const double wxdot =
-        (u.A1sq * E3 - k.A1A3 * E2 + A1 * E1 * jxz - A3 * E1 * jz + E2 * jxz * mt - E3 * jz * mt) / k.denom;
+        (u.A1sq * E3 - k.A1A3 * E2 + A1 * E1 * jxz - A3 * E1 * jz + E2 * jxz * mt - E3 * jz * mt) * k.rdenom;
     const double wzdot =
-        (k.A3sq * E2 - k.A1A2 * E1 - k.A1A3 * E3 + A3 * E1 * jxz - A2 * E2 * mt + E3 * jxz * mt) / k.denom;
+        (k.A3sq * E2 - k.A1A2 * E1 - k.A1A3 * E3 + A3 * E1 * jxz - A2 * E2 * mt + E3 * jxz * mt) * k.rdenom;
     const double udot = s0.wz * s0.v + k.inv_mt * inner;
     const double un = s0.u + udot * dt, vn = s0.v + vdot * dt, wzn = s0.wz + wzdot * dt;
     double sp, cp;
@@ -664,7 +664,7 @@ LFDM_HD void step18_adj(const PSet& P, const Derived& u, const K18& k, const S18
         omb[i] = o.omega[i];
     }
     // ---- chassis (computeVehRHS) ----
-    const double rden = 1.0 / k.denom;
+    const double rden = k.rdenom;
     const double Nvb = vdotb * rden, Nxb = wxdotb * rden, Nzb = wzdotb * rden;
     kb.denom -= (Nvb * vdot + Nxb * wxdot + Nzb * wzdot);
     double E1b = Nvb * (u.jxz2 - A2 * jz) + Nxb * (A1 * jxz - A3 * jz) + Nzb * (-k.A1A2 + A3 * jxz);

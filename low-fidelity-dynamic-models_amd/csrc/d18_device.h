@@ -30,7 +30,7 @@ struct X18 {
 struct K18 {
     double m, jz, jx, tqs, maxBrake;
     double Z1f, Z2fc, Z1r, Z2rc, Z4n;
-    double neg_mt, E3c, hm, A2, A3, inv_mt, two_hrc_m, denom, A1A2, A1A3, A3sq;
+    double neg_mt, E3c, hm, A2, A3, inv_mt, two_hrc_m, rdenom, A1A2, A1A3, A3sq;
     double qy1, qy2;  // InterpQ coefficients of dfy0 (2 w1 - w2/2, w1 - w2/2)
 };
 
@@ -58,7 +58,8 @@ LFDM_HD void k18_init(const PSet& P, double m, double jz, double jx, double dfy0
     k.A3 = A3;
     k.inv_mt = 1 / mt;
     k.two_hrc_m = 2. * hrc * m;
-    k.denom = (A2 * (A1 * A1) - 2. * A1 * A3 * P.jxz + jz * (A3 * A3) + mt * (P.jxz * P.jxz) - A2 * jz * mt);
+    // the chassis solve's determinant, kept as its reciprocal (each step multiplies by it)
+    k.rdenom = 1. / (A2 * (A1 * A1) - 2. * A1 * A3 * P.jxz + jz * (A3 * A3) + mt * (P.jxz * P.jxz) - A2 * jz * mt);
     k.A1A2 = A1 * A2;
     k.A1A3 = A1 * A3;
     k.A3sq = A3 * A3;
@@ -110,10 +111,10 @@ LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, doubl
     // computeTireLoads
     const double lat = s.vdot + s.wz * s.u;
     const double Z2f = k.Z2fc * lat;
-    const double Z3f = (P.krof * s.phi + P.brof * s.wx) / P.cf;
-    const double Z4 = (k.Z4n * (s.udot - s.wz * s.v)) / u.two_ab;
+    const double Z3f = (P.krof * s.phi + P.brof * s.wx) * u.inv_cf;
+    const double Z4 = (k.Z4n * (s.udot - s.wz * s.v)) * u.inv_two_ab;
     const double Z2r = k.Z2rc * lat;
-    const double Z3r = (P.kror * s.phi + P.bror * s.wx) / P.cr;
+    const double Z3r = (P.kror * s.phi + P.bror * s.wx) * u.inv_cr;
     double fz[4];
     fz[0] = k.Z1f - Z2f - Z3f - Z4;
     fz[1] = k.Z1f + Z2f + Z3f - Z4;
@@ -175,12 +176,12 @@ LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, doubl
                                                  k.two_hrc_m * s.wz * s.wx);
     const double A1 = u.A1, A2 = k.A2, A3 = k.A3, jz = k.jz, jxz = P.jxz, mt = -k.neg_mt;
     const double A3E1 = A3 * E1;
-    const double vdot = (E1 * u.jxz2 - k.A1A2 * E2 + A1 * E3 * jxz + A3 * E2 * jxz - A2 * E1 * jz - A3 * E3 * jz) /
-                        k.denom;
+    const double vdot = (E1 * u.jxz2 - k.A1A2 * E2 + A1 * E3 * jxz + A3 * E2 * jxz - A2 * E1 * jz - A3 * E3 * jz) *
+                        k.rdenom;
     const double wxdot =
-        (u.A1sq * E3 - k.A1A3 * E2 + A1 * E1 * jxz - A3E1 * jz + E2 * jxz * mt - E3 * jz * mt) / k.denom;
+        (u.A1sq * E3 - k.A1A3 * E2 + A1 * E1 * jxz - A3E1 * jz + E2 * jxz * mt - E3 * jz * mt) * k.rdenom;
     const double wzdot =
-        (k.A3sq * E2 - k.A1A2 * E1 - k.A1A3 * E3 + A3E1 * jxz - A2 * E2 * mt + E3 * jxz * mt) / k.denom;
+        (k.A3sq * E2 - k.A1A2 * E1 - k.A1A3 * E3 + A3E1 * jxz - A2 * E2 * mt + E3 * jxz * mt) * k.rdenom;
     // Half-implicit update (dof18_halfImplicit.cpp:262-305)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
