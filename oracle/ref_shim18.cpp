@@ -219,4 +219,23 @@ double ref18_run(void* h, double t0, int nsteps, const double* ctrl, int every, 
 // Runs the reference's own Solve() (driver file, end time from the last driver row).
 void ref18_solve(void* h) { static_cast<Ref18*>(h)->solver.Solve(); }
 
+// One IntegrateStepWithJacobian(t, throttle, steering, braking, true) from the current state: writes the
+// reference's central-difference Jacobians of the right-hand side, state [n][n] and controls
+// [n][2] (steering, throttle), row-major in its packY order, then advances one step as the reference
+// does.  Returns n (21/20 for TMeasy with/without converter, 13/12 for TMeasyNr).
+int ref18_step_jacobian(void* h, double t, double throttle, double steering, double braking, double* js,
+                        double* jc) {
+    Ref18* r = static_cast<Ref18*>(h);
+    r->solver.IntegrateStepWithJacobian(t, throttle, steering, braking, true);
+    const auto J = r->solver.GetJacobianState();
+    const auto C = r->solver.GetJacobianControls();
+    const int n = (int)J.size();
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) js[i * n + j] = J[i][j];
+        jc[2 * i] = C[i][0];
+        jc[2 * i + 1] = C[i][1];
+    }
+    return n;
+}
+
 }  // extern "C"

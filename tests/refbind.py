@@ -41,6 +41,8 @@ def _lib(dof):
         f("record_len").restype = i
         if dof == 18:
             f("set_record").argtypes = [p, p]
+            f("step_jacobian").argtypes = [p, d, d, d, d, p, p]
+            f("step_jacobian").restype = i
         if dof == 24:
             f("initialize").argtypes = [p, d]
         _libs[dof] = L
@@ -113,3 +115,11 @@ class RefSolver:
 class Ref18(RefSolver):
     def __init__(self, veh, tire, tire_nr=False, driver=None, dt=1e-3):
         super().__init__(18, veh, tire, tire_nr, driver, dt)
+
+    def step_jacobian(self, t, steering, throttle, braking):
+        """IntegrateStepWithJacobian from the current state: (J_state [n, n], J_controls [n, 2]) of the
+        right-hand side in the reference's packY order; the solver then stands one step later."""
+        js, jc = np.zeros(21 * 21), np.zeros(21 * 2)
+        n = self._f("step_jacobian")(self.h, float(t), float(throttle), float(steering), float(braking),
+                                     js.ctypes.data, jc.ctypes.data)
+        return js[: n * n].reshape(n, n).copy(), jc[: 2 * n].reshape(n, 2).copy()
