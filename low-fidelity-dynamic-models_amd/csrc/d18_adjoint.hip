@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W, W))
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int wave = uniform(slot >> 6);
     if (wave * kWave >= L.S) return;
-    adj_forward_lane(L, slot, L.sets[uniform(L.wave_set[wave])]);
+    adj_forward_lane(L, slot, pset_at(L.sets, uniform(L.wave_set[wave])));
 }
 
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(LFDM_ADJ_RECOMPUTE_WAVES,
@@ -49,7 +49,7 @@ adj_recompute(AdjLaunch L) {
     const int slot = (int)(id - (size_t)q * L.S);
     if (q >= L.seg_hi - L.seg_lo) return;
     __shared__ K18 ks[kThreads];
-    adj_recompute_lane(L, q, slot, L.sets[uniform(L.wave_set[slot >> 6])], ks[threadIdx.x]);
+    adj_recompute_lane(L, q, slot, pset_at(L.sets, uniform(L.wave_set[slot >> 6])), ks[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void adj_reverse(AdjLaunch L) {
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1))
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int wave = uniform(slot >> 6);
     if (wave * kWave >= L.S) return;
-    adj_reverse_lane(L, slot, L.sets[uniform(L.wave_set[wave])], ks[threadIdx.x], kbs[threadIdx.x]);
+    adj_reverse_lane(L, slot, pset_at(L.sets, uniform(L.wave_set[wave])), ks[threadIdx.x], kbs[threadIdx.x]);
 }
 
 }  // namespace

@@ -139,7 +139,7 @@ LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, doubl
     double fx[4], fy[4], My[4], xt[4], rStat[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const double d = i < 2 ? sd / cd : u.tan_delta_rear;
+        const double d = i < 2 ? qdiv(sd, cd) : u.tan_delta_rear;
         if (!NR)
             tire_tmeasy(t, u, k, d, fz[i], vsx[i], vsy[i], s.xe[i], s.ye[i], s.xedot[i], s.yedot[i], s.omega[i],
                         fx[i], fy[i], My[i], xt[i], rStat[i]);

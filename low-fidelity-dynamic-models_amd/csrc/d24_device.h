@@ -115,10 +115,10 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     // computeTireRHS (dof24.cpp:827-1121): load from the compression, radius from the attitude
     double fx[4], fy[4], fz[4], My[4];
     const double cc = cth * cph;
-    const double rcc = 1. / cc;
+    const double rcc = qdiv(1., cc);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const double d = i < 2 ? sd / cd : u.tan_delta_rear;
+        const double d = i < 2 ? qdiv(sd, cd) : u.tan_delta_rear;
         fz[i] = s.xt[i] * t.kt;
         s.rStat[i] = (t.r0 - s.xt[i]) * rcc;
         if (fz[i] <= 0) {
@@ -232,7 +232,7 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     s.wz = s.wz + wzdot * dt;
     // Euler-angle rates at the step's (old) theta and phi: tan(theta) and 1/cos(theta) from the
     // step's sine and cosine (same values, other rounding than tan() and two divisions)
-    const double rcth = 1. / cth;
+    const double rcth = qdiv(1., cth);
     const double tth = sth * rcth;
     const double dtheta = s.wy * cph - s.wz * sph;
     const double dpsi = (s.wy * sph * rcth) + (s.wz * cph * rcth);

@@ -26,6 +26,52 @@ constexpr double kHalfSqrt2 = 0.70710678118654757;              // sqrt(2.) / 2.
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Quotient and square root of the step code.  On the device: Newton-Raphson refinement of the
+// hardware reciprocal v_rcp_f64 (two iterations, then the quotient with one remainder correction,
+// Markstein's method) and of the reciprocal square root v_rsq_f64 (Goldschmidt-style coupled
+// iteration for sqrt(x) and 1 / (2 sqrt(x)), then two remainder corrections).  These are exactly the
+// operations of the compiler's correctly rounded expansion of `/` and sqrt without its exponent
+// range scaling (v_div_scale / v_div_fmas / v_div_fixup, v_ldexp) and special-operand fixups, so
+// they return the same bits as `/` and sqrt whenever no scaling is needed: operands and quotient of
+// qdiv finite, nonzero divisor, exponents within +-900 (a zero numerator can come back as a zero
+// of the other sign, which no later operation of the step can tell apart), and qsqrt's x finite and
+// >= 2^-767 or zero.  The step's operands are physical quantities far inside that
+// range, and every divisor it passes is bounded away from zero.  Without the scaling and fixups a
+// division costs 8 VALU instructions instead of 11 plus a VCC dependency, a square root 12 instead
+// of 17.  Host builds (the adjoint's finite-difference checks) use `/` and sqrt.
+LFDM_HD double qdiv(double n, double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    const double q = n * r;
+    const double rem = __builtin_fma(-d, q, n);
+    return __builtin_fma(rem, r, q);
+#else
+    return n / d;
+#endif
+}
+
+LFDM_HD double qsqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    double d = __builtin_fma(-g, g, x);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return x == 0.0 ? x : g;  // v_rsq_f64(0) is +inf
+#else
+    return sqrt(x);
+#endif
+}
+
 LFDM_HD double sgn(double v) { return (double)((0.0 < v) - (v < 0.0)); }
 
 LFDM_HD double clampd(double v, double lo, double hi) {
@@ -45,7 +91,7 @@ LFDM_HD double sine_step(double x, double x1, double y1, double x2, double y2) {
     if (x <= x1) return y1;
     if (x >= x2) return y2;
     const double dx = x2 - x1, dy = y2 - y1;
-    return y1 + dy * (x - x1) / dx - (dy / k2Pi) * sin(k2Pi * (x - x1) / dx);
+    return y1 + qdiv(dy * (x - x1), dx) - qdiv(dy, k2Pi) * sin(qdiv(k2Pi * (x - x1), dx));
 }
 
 // Clamped piecewise-linear lookup (utils.cpp:109-125). `m` is wave-uniform, so the knot scan
@@ -60,7 +106,7 @@ LFDM_HD double map_y(const Map& m, double x) {
     r += 1;
     const MapXY L = m.e[r - 1];
     const MapXY R = m.e[r];
-    const double mbar = (x - L.x) / (R.x - L.x);
+    const double mbar = qdiv(x - L.x, R.x - L.x);
     return L.y + mbar * (R.y - L.y);
 }
 
@@ -76,7 +122,7 @@ LFDM_HD double map_y_scaled(const Map& m, double x, double sx, double sy1, doubl
     const MapXY L = m.e[r - 1];
     const MapXY R = m.e[r];
     const double lx = L.x * sx, rx = R.x * sx, ly = L.y * sy1 * sy2, ry = R.y * sy1 * sy2;
-    const double mbar = (x - lx) / (rx - lx);
+    const double mbar = qdiv(x - lx, rx - lx);
     return ly + mbar * (ry - ly);
 }
 
@@ -95,7 +141,7 @@ LFDM_HD void driver_input(double t, const DrvRow* __restrict__ rows, int n, doub
     while (r < n - 1 && rows[r].t < t) ++r;
     const DrvRow L = rows[r - 1];
     const DrvRow R = rows[r];
-    const double tbar = (t - L.t) / (R.t - L.t);
+    const double tbar = qdiv(t - L.t, R.t - L.t);
     s = L.steering + tbar * (R.steering - L.steering);
     th = L.throttle + tbar * (R.throttle - L.throttle);
     b = L.braking + tbar * (R.braking - L.braking);

@@ -14,36 +14,36 @@ LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, d
                                            double& f) {
     double df0loc = 0.0;
     if (sm > 0.0) {
-        const double c = 2.0 * fm / sm;
+        const double c = qdiv(2.0 * fm, sm);
         df0loc = (c < df0) ? df0 : c;
     }
     double fos;
     if (s > 0.0 && df0loc > 0.0) {
         if (s > ss) {
             f = fs;
-            fos = f / s;
+            fos = qdiv(f, s);
         } else if (s < sm) {
-            const double p = df0loc * sm / fm - 2.0;
-            const double sn = s / sm;
-            const double rdn = 1.0 / (1.0 + (sn + p) * sn);
+            const double p = qdiv(df0loc * sm, fm) - 2.0;
+            const double sn = qdiv(s, sm);
+            const double rdn = qdiv(1.0, 1.0 + (sn + p) * sn);
             f = df0loc * sm * sn * rdn;
             fos = df0loc * rdn;
         } else {
-            const double q = fm / sm;
-            const double a = q * q / (df0loc * sm);
-            const double sstar = sm + (fm - fs) / (a * (ss - sm));
+            const double q = qdiv(fm, sm);
+            const double a = qdiv(q * q, df0loc * sm);
+            const double sstar = sm + qdiv(fm - fs, a * (ss - sm));
             if (sstar <= ss) {
                 if (s <= sstar) {
                     f = fm - a * (s - sm) * (s - sm);
                 } else {
-                    const double b = a * (sstar - sm) / (ss - sstar);
+                    const double b = qdiv(a * (sstar - sm), ss - sstar);
                     f = fs + b * (ss - s) * (ss - s);
                 }
             } else {
-                const double sn = (s - sm) / (ss - sm);
+                const double sn = qdiv(s - sm, ss - sm);
                 f = fm - (fm - fs) * sn * sn * (3.0 - 2.0 * sn);
             }
-            fos = f / s;
+            fos = qdiv(f, s);
         }
     } else {
         f = 0.0;
@@ -87,7 +87,7 @@ LFDM_HD void curve_at(const TireP& t, const Derived& u, double qy1, double qy2, 
 //    away from the overflow and underflow that hypot's scaling guards against;
 //  - divisions by a shared denominator (vta, hsxn, hsyn, sc, sxm + sym, rx + ry, pn, kt) as
 //    multiplications by its reciprocal.
-LFDM_HD double hyp(double a, double b) { return sqrt(a * a + b * b); }
+LFDM_HD double hyp(double a, double b) { return qsqrt(a * a + b * b); }
 
 // Intermediate values of the TMeasy force core (dof18.cpp:412-539; the same code is
 // dof11.cpp:335-472 and dof24.cpp:851-987) for a tire on the ground (fz > 0) with static radius
@@ -121,30 +121,30 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
     w.som = sgn(omega);
     w.rw = w.r_eff * w.aom;
     w.vta = w.rw + 0.01;
-    const double rvta = 1. / w.vta;
+    const double rvta = qdiv(1., w.vta);
     w.sx = -w.vsx2 * rvta;
     const double r = vsy * rvta;  // tan(atan2(vsy, vta))
-    w.sy = -((r - tan_delta) / (1. + r * tan_delta));
+    w.sy = -qdiv(r - tan_delta, 1. + r * tan_delta);
     w.clip = fz > t.pnmax;
     w.fzc = w.clip ? t.pnmax : fz;
     const Curve& c = w.c;
     curve_at(t, u, qy1, qy2, w.fzc, w.c);
     w.fzn = w.fzc * u.inv_pn;
-    w.rx = c.fxm / c.dfx0;
-    w.ry = c.fym / c.dfy0;
+    w.rx = qdiv(c.fxm, c.dfx0);
+    w.ry = qdiv(c.fym, c.dfy0);
     w.ssum = c.sxm + c.sym;
     w.rsum = w.rx + w.ry;
-    w.rssum = 1. / w.ssum;
-    w.rrsum = 1. / w.rsum;
+    w.rssum = qdiv(1., w.ssum);
+    w.rrsum = qdiv(1., w.rsum);
     w.hsxn = c.sxm * w.rssum + w.rx * w.rrsum;
     w.hsyn = c.sym * w.rssum + w.ry * w.rrsum;
-    w.rhsxn = 1. / w.hsxn;
-    w.rhsyn = 1. / w.hsyn;
+    w.rhsxn = qdiv(1., w.hsxn);
+    w.rhsyn = qdiv(1., w.hsyn);
     w.sxn = w.sx * w.rhsxn;
     w.syn = w.sy * w.rhsyn;
     w.sc = hyp(w.sxn, w.syn);
     if (w.sc > 0) {
-        const double rsc = 1. / w.sc;
+        const double rsc = qdiv(1., w.sc);
         w.ca = w.sxn * rsc;
         w.sa = w.syn * rsc;
     } else {
@@ -178,8 +178,8 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
     w.denx = w.vtxs * t.dx + w.fos;
     w.numy = -w.vtys * t.cy * ye - w.fos * w.vy;
     w.deny = w.vtys * t.dy + w.fos;
-    w.xedot = w.numx / w.denx;
-    w.yedot = w.numy / w.deny;
+    w.xedot = qdiv(w.numx, w.denx);
+    w.yedot = qdiv(w.numy, w.deny);
     w.fxdyn = t.dx * w.xedot + t.cx * xe;
     w.fydyn = t.dy * w.yedot + (t.cy * ye);
     w.fxr = xe * t.cx + w.xedot * t.dx;
@@ -221,25 +221,25 @@ LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double
     const double r_eff = rdynco * t.r0 + (1. - rdynco) * rStat;
     vsx = vsx - (omega * r_eff);
     const double vta = r_eff * fabs(omega) + 0.01;
-    double Fx0 = tanh(-2.0 * vsx / t.vcoulomb) * fz * t.mu;
-    double Fy0 = tanh(-2.0 * vsy / t.vcoulomb) * fz * t.mu;
+    double Fx0 = tanh(qdiv(-2.0 * vsx, t.vcoulomb)) * fz * t.mu;
+    double Fy0 = tanh(qdiv(-2.0 * vsy, t.vcoulomb)) * fz * t.mu;
     const double F0 = hyp(Fx0, Fy0);
     if (F0 > fz * t.mu) {
-        const double fsc = fz * t.mu / F0;
+        const double fsc = qdiv(fz * t.mu, F0);
         Fx0 = Fx0 * fsc;
         Fy0 = Fy0 * fsc;
     }
-    const double rvta = 1. / vta;
+    const double rvta = qdiv(1., vta);
     const double sx = -vsx * rvta;
     const double r = vsy * rvta;  // tan(atan2(vsy, vta))
-    const double sy = -((r - tan_delta) / (1. + r * tan_delta));
+    const double sy = -qdiv(r - tan_delta, 1. + r * tan_delta);
     if (fz > t.pnmax) fz = t.pnmax;
     Curve c;
     curve_at(t, u, qy1, qy2, fz, c);
     const double frblend = sine_step(fabs(vsx_in), t.frblend_begin, 0., t.frblend_end, 1.);
     const double sc = hyp(sx, sy);
     double ca, sa;
-    const double rsc = sc > 0 ? 1. / sc : 0.;
+    const double rsc = sc > 0 ? qdiv(1., sc) : 0.;
     if (sc > 0) {
         ca = sx * rsc;
         sa = sy * rsc;
@@ -276,7 +276,7 @@ LFDM_HD void diff_split(double torque, double sl, double sr, double& tl, double&
         bias = max_bias;
     else if (diff > 0.25)
         bias = 4 * (max_bias - 1) * diff + (2 - max_bias);
-    const double alpha = bias / (1 + bias);
+    const double alpha = qdiv(bias, 1 + bias);
     const double slow = alpha * torque;
     const double fast = torque - slow;
     const bool left_slow = fabs(sl) < fabs(sr);
@@ -308,12 +308,12 @@ LFDM_HD void powertrain4(const PSet& P, const Derived& u, double tqs, double max
     const double gr = pick(P.gears, P.ngears, gear);
     double shift_w;
     if (P.tcbool) {
-        const double omega_out = 0.25 * osum / gr;
+        const double omega_out = qdiv(0.25 * osum, gr);
         const double omega_in = crank;
         double sr = 0;
         bool rev = false;
         if (!(omega_out < 1e-9 || omega_in < 1e-9)) {
-            sr = omega_out / omega_in;
+            sr = qdiv(omega_out, omega_in);
             if (sr > 1.) {
                 sr = 1. - (sr - 1.);
                 rev = true;
@@ -322,17 +322,17 @@ LFDM_HD void powertrain4(const PSet& P, const Derived& u, double tqs, double max
         }
         const double cf = map_y(P.cfmap, sr);
         const double tr = map_y(P.trmap, sr);
-        const double q = omega_in / cf;
+        const double q = qdiv(omega_in, cf);
         double torque_in = -(q * q);
         if (rev) torque_in = -torque_in;
         const double torque_out = rev ? -torque_in : -tr * torque_in;
-        torque_t = torque_out / gr;
+        torque_t = qdiv(torque_out, gr);
         if (vel_u < 1e-9 && torque_t < 0) torque_t = 0;
         dcrank = u.inv_ci * (drive_torque(P, tqs, throttle, crank) + torque_in);
         shift_w = omega_out;
     } else {
-        crank = 0.25 * osum / gr;
-        torque_t = drive_torque(P, tqs, throttle, crank) / gr;
+        crank = qdiv(0.25 * osum, gr);
+        torque_t = qdiv(drive_torque(P, tqs, throttle, crank), gr);
         if (vel_u < 1e-9 && torque_t < 0) torque_t = 0;
         shift_w = crank;
     }

@@ -349,7 +349,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
         if (wsteps == 0 && L.n_out == 0) return;
     }
     const int set = uniform(L.wave_set[wave]);
-    const PSet& P = L.sets[set];
+    const PSet& P = pset_at(L.sets, set);
     const size_t S = (size_t)L.S;
 
     typename M::S s;

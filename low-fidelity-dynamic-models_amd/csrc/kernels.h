@@ -12,6 +12,13 @@
 
 namespace lfdm {
 
+// Parameter set i of a launch.  Sets are never written while a kernel runs, so they are read
+// through the constant address space: reads at a wave-uniform address then become scalar loads into
+// SGPRs instead of vector loads into VGPR pairs (which also lowers the step's VGPR pressure).
+__device__ __forceinline__ const PSet& pset_at(const PSet* sets, int i) {
+    return *(const PSet*)((const __attribute__((address_space(4))) PSet*)sets + i);
+}
+
 struct Launch {
     const PSet* sets;       // parameter sets
     const int* wave_set;   // set index of every 64-slot wavefront

@@ -1,0 +1,110 @@
+// GPU check of the step code's quotient and square root (qdiv, qsqrt in csrc/dev_common.h) against
+// the compiler's IEEE `/` and sqrt on the device and against the host's correctly rounded results.
+//
+// Operands: random signs, mantissas and exponents over the range the helpers are specified for
+// (quotients of operands with exponents in [-450, 450], square roots of x in [2^-700, 2^700]), plus
+// operands shaped like the step's (ratios of numbers in [1e-6, 1e6], zero numerators, squares of
+// zero).  Prints the mismatch counts; exits non-zero if qdiv or qsqrt differs from the device's `/` or
+// sqrt (the device-versus-host counts are information).
+// Usage: arith_check [n]
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "../../low-fidelity-dynamic-models_amd/csrc/dev_common.h"
+
+using lfdm::dev::qdiv;
+using lfdm::dev::qsqrt;
+
+__global__ void arith(const double* a, const double* b, const double* x, int n, double* q_ieee, double* q_fast,
+                      double* s_ieee, double* s_fast) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    q_ieee[i] = a[i] / b[i];
+    q_fast[i] = qdiv(a[i], b[i]);
+    s_ieee[i] = sqrt(x[i]);
+    s_fast[i] = qsqrt(x[i]);
+}
+
+static bool same(double p, double q) { return std::memcmp(&p, &q, sizeof p) == 0; }
+
+#define CHECK(call)                                                                  \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            std::fprintf(stderr, "%s: %s\n", #call, hipGetErrorString(e_));          \
+            return 2;                                                                \
+        }                                                                            \
+    } while (0)
+
+int main(int argc, char** argv) {
+    const int n = argc > 1 ? std::atoi(argv[1]) : (1 << 22);
+    if (n <= 0) return 2;
+    std::mt19937_64 rng(2024);
+    std::uniform_real_distribution<double> mant(1.0, 2.0);
+    std::uniform_int_distribution<int> e450(-450, 450), e700(-700, 700), sign(0, 1);
+    std::uniform_real_distribution<double> phys(-6.0, 6.0);
+    std::vector<double> a(n), b(n), x(n);
+    for (int i = 0; i < n; ++i) {
+        if (i % 4 == 0) {  // step-shaped operands
+            a[i] = (sign(rng) ? -1 : 1) * std::pow(10.0, phys(rng));
+            b[i] = (sign(rng) ? -1 : 1) * std::pow(10.0, phys(rng));
+            const double r = std::pow(10.0, phys(rng)), s = std::pow(10.0, phys(rng));
+            x[i] = r * r + s * s;
+            if (i % 64 == 0) a[i] = (i % 128 == 0) ? 0.0 : -0.0;
+            if (i % 256 == 4) x[i] = 0.0;
+        } else {
+            a[i] = (sign(rng) ? -1 : 1) * std::ldexp(mant(rng), e450(rng));
+            b[i] = (sign(rng) ? -1 : 1) * std::ldexp(mant(rng), e450(rng));
+            x[i] = std::ldexp(mant(rng), e700(rng));
+        }
+    }
+    const size_t bytes = sizeof(double) * (size_t)n;
+    double *da, *db, *dx, *q1, *q2, *s1, *s2;
+    CHECK(hipMalloc(&da, bytes));
+    CHECK(hipMalloc(&db, bytes));
+    CHECK(hipMalloc(&dx, bytes));
+    CHECK(hipMalloc(&q1, bytes));
+    CHECK(hipMalloc(&q2, bytes));
+    CHECK(hipMalloc(&s1, bytes));
+    CHECK(hipMalloc(&s2, bytes));
+    CHECK(hipMemcpy(da, a.data(), bytes, hipMemcpyHostToDevice));
+    CHECK(hipMemcpy(db, b.data(), bytes, hipMemcpyHostToDevice));
+    CHECK(hipMemcpy(dx, x.data(), bytes, hipMemcpyHostToDevice));
+    const int block = 256, grid = (n + block - 1) / block;
+    hipLaunchKernelGGL(arith, dim3(grid), dim3(block), 0, 0, da, db, dx, n, q1, q2, s1, s2);
+    CHECK(hipGetLastError());
+    CHECK(hipDeviceSynchronize());
+    std::vector<double> hq1(n), hq2(n), hs1(n), hs2(n);
+    CHECK(hipMemcpy(hq1.data(), q1, bytes, hipMemcpyDeviceToHost));
+    CHECK(hipMemcpy(hq2.data(), q2, bytes, hipMemcpyDeviceToHost));
+    CHECK(hipMemcpy(hs1.data(), s1, bytes, hipMemcpyDeviceToHost));
+    CHECK(hipMemcpy(hs2.data(), s2, bytes, hipMemcpyDeviceToHost));
+    for (double* p : {da, db, dx, q1, q2, s1, s2}) CHECK(hipFree(p));
+
+    long qd = 0, qh = 0, sd = 0, sh = 0, zsign = 0;
+    for (int i = 0; i < n; ++i) {
+        const double q = a[i] / b[i], s = std::sqrt(x[i]);
+        if (!same(hq2[i], hq1[i])) {
+            // a zero numerator: the fast path returns +0 where IEEE division keeps the sign
+            if (a[i] == 0.0 && hq2[i] == 0.0 && hq1[i] == 0.0) ++zsign;
+            else if (qd++ < 5) std::printf("qdiv(%a, %a) = %a, '/' gives %a\n", a[i], b[i], hq2[i], hq1[i]);
+        }
+        if (!same(hq1[i], q)) ++qh;
+        if (!same(hs2[i], hs1[i]) && sd++ < 5) std::printf("qsqrt(%a) = %a, sqrt gives %a\n", x[i], hs2[i], hs1[i]);
+        if (!same(hs1[i], s)) ++sh;
+    }
+    std::printf("n = %d\n", n);
+    std::printf("qdiv vs device '/': %ld mismatches (+ %ld signed-zero quotients of zero numerators)\n", qd, zsign);
+    std::printf("device '/' vs host: %ld mismatches\n", qh);
+    std::printf("qsqrt vs device sqrt: %ld mismatches\n", sd);
+    std::printf("device sqrt vs host: %ld mismatches\n", sh);
+    const bool ok = qd == 0 && sd == 0;
+    std::printf("%s\n", ok ? "ALL OK" : "MISMATCH");
+    return ok ? 0 : 1;
+}
