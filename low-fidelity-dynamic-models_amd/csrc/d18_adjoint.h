@@ -82,8 +82,8 @@ LFDM_HD void hypot_adj(double a, double b, double h, double hb, double& ab, doub
 }
 
 // The reverse sweeps below multiply by reciprocals of repeated denominators instead of dividing
-// each time (an FP64 division is ~10 instructions): the adjoint's last bits change, the forward
-// values and branch choices it recomputes do not.
+// each time (a correctly rounded FP64 quotient is 8 instructions, a product 1): the adjoint's last
+// bits change, the forward values and branch choices it recomputes do not.
 
 // Adjoint of tmxy_combined (dof18.cpp:247-288): bars of (s, df0, sm, fm, ss, fs) from bars of (f, fos).
 LFDM_HD void tmxy_adj(double s, double df0, double sm, double fm, double ss, double fs, double fb, double fosb,
