@@ -12,8 +12,10 @@ steps) in one launch, with every input already resident in HBM.
 Prints one JSON line; `roofline` uses the algorithmic FP64 FLOPs per vehicle-step counted by the
 oracle's counting scalar on the same workload (+ - * / sqrt count 1, sin cos tan atan2 tanh
 count 20, hypot counts 4) and the device time of the integration launch from HIP events on the
-solver's stream.  `cpu_baseline` times the oracle (the CPU restatement, bit-identical to the
-reference solver) with OpenMP over vehicles on a bounded sample of the same workload.  The default
+solver's stream.  `cpu_baseline` times the reference's own CPU solver (compiled from its sources
+into oracle/_ref, one solver object per vehicle, Python threads) on a bounded sample of the same
+workload, with the oracle (the CPU restatement, bit-identical to the reference solver, OpenMP over
+vehicles) beside it under `port`; without oracle/_ref the oracle is the baseline.  The default
 18-DOF line also carries `other_models` (the 24-DOF configuration, 500,000 vehicles over the same
 horizon, and the 11-DOF model on the headline fleet), `adjoint` (the differentiation form on 65,536
 vehicles x 5,000 steps, checkpoints every 100, with 20 MPC iterations) and `mixed_fleet` (the
@@ -84,6 +86,76 @@ def cpu_baseline(dof, flat, n_sample, nsteps, threads):
           init=init1, threads=1)
     d1 = time.perf_counter() - t1
     return n_sample * nsteps / dt, n1 * nsteps / d1, dt
+
+
+def cpu_reference(n_sample, nsteps, threads, n_single=32):
+    """The reference's own 18-DOF CPU solver (d18SolverHalfImplicit::IntegrateStep, compiled from the
+    reference sources into oracle/_ref/libref18.so by oracle/Makefile) on a sample of the knot_fleet
+    workload: one solver object per vehicle with that vehicle's mass, yaw inertia and cornering
+    stiffness, per-step controls from the vehicle's knot program.  Vehicles are spread over `threads`
+    Python threads (ctypes releases the GIL during each call); construction and control expansion
+    happen before the timed region.  Returns (vehicle-steps/s, single-core vehicle-steps/s, seconds)
+    or None where oracle/_ref was not built."""
+    import ctypes
+    import threading
+
+    from lfdm_amd import OVERRIDE_FIELDS
+    from lfdm_amd import workloads as W
+
+    path = os.path.join(ROOT, "oracle", "_ref", "libref18.so")
+    if not os.path.exists(path):
+        return None
+    L = ctypes.CDLL(path)
+    p, d, i, s = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_char_p
+    L.ref18_create.restype = p
+    L.ref18_create.argtypes = [s, s, i, s]
+    L.ref18_destroy.argtypes = [p]
+    L.ref18_set_step.argtypes = [p, d]
+    L.ref18_set_param.restype = i
+    L.ref18_set_param.argtypes = [p, s, d]
+    L.ref18_run.restype = d
+    L.ref18_run.argtypes = [p, d, i, p, i, p]
+    hm = os.path.join(W.DATA, "18dof", "HMMWV")
+    veh, tire = os.fsencode(f"{hm}/vehicle.json"), os.fsencode(f"{hm}/tmeasy.json")
+    flat = hmmwv_params(18)
+    t = W.time_grid(nsteps, 1e-3)
+
+    def prepare(n, seed):
+        fl = W.knot_fleet(n, flat, seed=seed, dof=18)
+        hs, cs = [], []
+        for v in range(n):
+            h = L.ref18_create(veh, tire, 0, None)
+            L.ref18_set_step(h, 1e-3)
+            for f, name in enumerate(OVERRIDE_FIELDS):
+                if L.ref18_set_param(h, name.encode(), float(fl["ovr"][f, v])) != 0:
+                    raise RuntimeError(f"reference solver rejects parameter {name}")
+            hs.append(h)
+            cs.append(np.ascontiguousarray(
+                W.knot_controls(t, fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"][:, v:v + 1])[:, :, 0]))
+        return hs, cs
+
+    def timed(hs, cs, nthreads):
+        def work(k):
+            for v in range(k, len(hs), nthreads):
+                L.ref18_run(hs[v], 0.0, nsteps, cs[v].ctypes.data, 0, None)
+
+        pool = [threading.Thread(target=work, args=(k,)) for k in range(nthreads)]
+        t0 = time.perf_counter()
+        for th in pool:
+            th.start()
+        for th in pool:
+            th.join()
+        dt = time.perf_counter() - t0
+        for h in hs:
+            L.ref18_destroy(h)
+        return dt
+
+    hs, cs = prepare(n_sample, 2)
+    secs = timed(hs, cs, threads)
+    del cs
+    hs1, cs1 = prepare(n_single, 3)
+    secs1 = timed(hs1, cs1, 1)
+    return n_sample * nsteps / secs, n_single * nsteps / secs1, secs
 
 
 def adjoint_cost(flat, n=65536, T=5000, K=100, mpc_iterations=20):
@@ -317,6 +389,8 @@ def main():
     ap.add_argument("--horizon-steps", type=int, default=10000)
     ap.add_argument("--tpb", type=int, default=256)
     ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--ref-sample", type=int, default=4096,
+                    help="vehicles of the reference CPU solver's timed sample (oracle/_ref, when built)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation-form measurement")
     ap.add_argument("--adjoint-only", action="store_true", help="only the differentiation-form measurement")
@@ -371,10 +445,19 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         rate, rate1, secs = cpu_baseline(dof, flat, args.cpu_sample, H, threads)
-        cpu = {"value": rate, "unit": "vehicle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"knot_fleet, {args.cpu_sample} vehicles x {H} steps ({secs:.1f} s), "
-                         f"oracle (bit-identical CPU restatement), OpenMP over vehicles",
-               "single_core": rate1, "cpu": cpu_model()}
+        port = {"value": rate, "unit": "vehicle-steps/s", "cores": threads, "kind": "port",
+                "sample": f"knot_fleet, {args.cpu_sample} vehicles x {H} steps ({secs:.1f} s), "
+                          f"oracle (bit-identical CPU restatement), OpenMP over vehicles",
+                "single_core": rate1, "cpu": cpu_model()}
+        ref = cpu_reference(args.ref_sample, H, threads) if dof == 18 else None
+        if ref is None:
+            cpu = port
+        else:
+            cpu = {"value": ref[0], "unit": "vehicle-steps/s", "cores": threads, "kind": "reference",
+                   "sample": f"knot_fleet, {args.ref_sample} vehicles x {H} steps ({ref[2]:.1f} s), the reference's "
+                             f"own CPU solver (d18SolverHalfImplicit::IntegrateStep compiled from its sources into "
+                             f"oracle/_ref), one solver object per vehicle, {threads} threads",
+                   "single_core": ref[1], "cpu": cpu_model(), "port": port}
     metric = ("18-DOF FP64 vehicle-steps/s at 1M vehicles; adjoint/forward cost ratio" if dof == 18 else
               f"{dof}-DOF FP64 vehicle-steps/s")
     other = None
