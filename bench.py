@@ -462,7 +462,7 @@ def main():
               f"{dof}-DOF FP64 vehicle-steps/s")
     other = None
     if dof == 18 and not args.no_other_models:
-        # BASELINE's 24-DOF configuration (500,000 vehicles, 10 s) and the 11-DOF model on the headline fleet
+        # the 24-DOF configuration (500,000 vehicles, 10 s) and the 11-DOF model on the headline fleet
         other = {}
         for d_, n_ in ((24, 500_000), (11, 1_000_000)):
             v_, _, k_ = knot_fleet_rate(d_, hmmwv_params(d_), n_, H, 2, 1, args.tpb)
