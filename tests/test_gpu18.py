@@ -104,6 +104,39 @@ def test_reference_gpu_fleet_test(golden18):
     s.close()
 
 
+def test_heterogeneous_construct_groups(golden18):
+    """Three Construct groups of different vehicles in one handle (HMMWV on acc3, Sedan RWD on acc3,
+    audi on double_lane4; 70 / 50 / 90 vehicles, so every group ends in a padded wavefront), solved
+    together for 10 s: each group ends where the reference's own run of that vehicle ends (1e-9,
+    gear exact; dOmega_crank, which the reference leaves uninitialized without a torque converter,
+    masked as in the golden cases), identical vehicles bit-identical."""
+    say("[gpu18] heterogeneous Construct groups")
+    d = os.path.join(DATA, "18dof")
+    inp = os.path.join(DATA, "input")
+    groups = (("HMMWV/vehicle.json", "HMMWV/tmeasy.json", "acc3.txt", 70, "hmmwv_acc3"),
+              ("Sedan/vehicle_RWD.json", "Sedan/tmeasy.json", "acc3.txt", 50, "sedan_rwd_acc3"),
+              ("audi/vehicle.json", "audi/tmeasy.json", "double_lane4.txt", 90, "audi_dl4"))
+    total = sum(g[3] for g in groups)
+    s = SolverHalfImplicitGPU(18, total)
+    for veh, tire, drv, n, _ in groups:
+        s.Construct(f"{d}/{veh}", f"{d}/{tire}", n, os.path.join(inp, drv))
+    s.Initialize(s.zero_record(), total)
+    s.SetEndTime(10.0)
+    s.Solve()
+    allst = s.get_states()
+    s.close()
+    lo = 0
+    for _, _, _, n, case in groups:
+        got = masked(case, golden18, allst[lo:lo + n])
+        ref = masked(case, golden18, golden18[f"{case}/final"][0])
+        err = final_rel(got, masked(case, golden18, golden18[f"{case}/traj"][:, 0]), ref).max()
+        say(f"[gpu18] {case} group: max rel {err:.3e}")
+        assert err <= TOL
+        np.testing.assert_array_equal(got[:, -1], np.full(n, ref[-1]))
+        assert np.all(got == got[0])
+        lo += n
+
+
 def test_ramp_sine_1024_vs_oracle(golden18):
     """The 1,024-vehicle ramp_sine workload (10 s) against the oracle at 1e-9."""
     import lfdm_oracle as O
