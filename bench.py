@@ -223,7 +223,7 @@ def hmmwv_params(dof):
                                      f"{d}/suspension.json" if dof == 24 else None)
 
 
-def knot_fleet_rate(dof, flat, n, H, steps, warmup, tpb=256):
+def knot_fleet_rate(dof, flat, n, H, steps, warmup, tpb=64):
     """knot_fleet on n vehicles of the model, H solver steps per bench step (one launch), inputs
     resident in HBM: (vehicle-steps/s over the timed steps, their wall time, mean device time of a
     launch from HIP events on the solver's stream).  24-DOF vehicles start at U(0, 20) m/s."""
@@ -387,7 +387,7 @@ def main():
                          "at U(0, 20) m/s)")
     ap.add_argument("--vehicles", type=int, default=1_000_000)
     ap.add_argument("--horizon-steps", type=int, default=10000)
-    ap.add_argument("--tpb", type=int, default=256)
+    ap.add_argument("--tpb", type=int, default=64)
     ap.add_argument("--cpu-sample", type=int, default=32768)
     ap.add_argument("--ref-sample", type=int, default=4096,
                     help="vehicles of the reference CPU solver's timed sample (oracle/_ref, when built)")
