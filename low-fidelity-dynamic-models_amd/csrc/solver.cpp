@@ -134,6 +134,10 @@ struct lfdm_solver {
     size_t out_steps_cap = 0;
     double* d_out = nullptr;
     size_t out_cap = 0;
+    // sampled-vehicle output: the columns of the written vehicles, gathered on the device
+    double* d_outc = nullptr;
+    size_t outc_cap = 0;
+    int* d_which = nullptr;
     double* d_rec = nullptr;
     size_t rec_cap = 0;
     std::vector<int> outs_host;  // output steps of the launch in flight (source of an async copy)
@@ -502,16 +506,20 @@ void csv_header(lfdm_solver* h) {
     }
 }
 
-void csv_rows(lfdm_solver* h, const std::vector<double>& host, unsigned rows) {
+// Appends `rows` output samples to the CSV streams.  With every vehicle written, `host` holds
+// [sample][channel][vehicle] rows; with sampled vehicles it holds, per launch of `dct` samples,
+// [written vehicle][sample][channel] (the device gathers only their columns).
+void csv_rows(lfdm_solver* h, const std::vector<double>& host, unsigned rows, unsigned dct) {
     const unsigned nw = h->store_all ? h->total : h->num_outs;
     const size_t N = h->total;
     const int nc = out_channels(h->dof);
     for (unsigned i = 0; i < nw; ++i) {
-        const unsigned v = h->store_all ? i : h->which_outs[i];
         std::ostringstream& o = h->csv[i];
         for (unsigned r = 0; r < rows; ++r) {
-            const size_t off = (size_t)r * N * nc;
-            for (int c = 0; c < nc; ++c) o << host[off + N * c + v] << ",";
+            const size_t off = h->store_all ? (size_t)r * N * nc + i
+                                            : (((size_t)(r / dct) * nw + i) * dct + r % dct) * nc;
+            const size_t stride = h->store_all ? N : 1;
+            for (int c = 0; c < nc; ++c) o << host[off + stride * c] << ",";
             o << std::endl;
         }
     }
@@ -648,6 +656,8 @@ int lfdm_destroy(lfdm_solver* h) {
     dfree(h->d_steps);
     dfree(h->d_out_steps);
     dfree(h->d_out);
+    dfree(h->d_outc);
+    dfree(h->d_which);
     dfree(h->d_rec);
     dfree(h->d_horizon);
     dfree(h->d_wave_hmax);
@@ -853,12 +863,21 @@ int lfdm_solve(lfdm_solver* h) {
     const size_t nc = (size_t)out_channels(h->dof);
     unsigned dct = 0, hct = 0;
     std::vector<double> host;
+    // columns copied to the host per output sample: every vehicle, or only the written ones
+    const size_t W = h->store_all ? N : (size_t)h->num_outs;
     if (h->output) {
         dct = (unsigned)std::ceil(h->ksim / h->dtout);
         hct = (unsigned)std::ceil(h->host_dump / h->dtout);
         if (int rc = ensure_buffer(h->d_out, h->out_cap, (size_t)dct * nc * N)) return rc;
         HIPCHK(hipMemsetAsync(h->d_out, 0, (size_t)dct * nc * N * sizeof(double), h->stream));
-        host.assign((size_t)std::max(hct, dct) * nc * N, 0.0);
+        if (!h->store_all) {
+            if (int rc = ensure_buffer(h->d_outc, h->outc_cap, (size_t)dct * nc * W)) return rc;
+            std::vector<int> which(h->which_outs.begin(), h->which_outs.end());
+            dfree(h->d_which);
+            HIPCHK(hipMalloc((void**)&h->d_which, std::max<size_t>(W, 1) * sizeof(int)));
+            HIPCHK(hipMemcpyAsync(h->d_which, which.data(), W * sizeof(int), hipMemcpyHostToDevice, h->stream));
+        }
+        host.assign((size_t)std::max(hct, dct) * nc * W, 0.0);
         for (auto& s : h->csv) { s.str(""); s.clear(); }
         csv_header(h);
     }
@@ -884,14 +903,19 @@ int lfdm_solve(lfdm_solver* h) {
         current = kend;
         since += h->ksim;
         if (h->output) {
-            const size_t filled = N * nc * dct * launches;
-            if (host.size() < filled + (size_t)dct * nc * N) host.resize(filled + (size_t)dct * nc * N, 0.0);
-            HIPCHK(hipMemcpyAsync(host.data() + filled, h->d_out, (size_t)dct * nc * N * sizeof(double),
-                                  hipMemcpyDeviceToHost, h->stream));
+            const size_t chunk = (size_t)dct * nc * W;
+            const size_t filled = chunk * launches;
+            if (host.size() < filled + chunk) host.resize(filled + chunk, 0.0);
+            const double* src = h->d_out;
+            if (!h->store_all) {  // [written vehicle][sample][channel] of this launch
+                HIPCHK(launch_gather(h->d_out, (int)N, (int)(dct * nc), h->d_which, (int)W, h->d_outc, h->stream));
+                src = h->d_outc;
+            }
+            HIPCHK(hipMemcpyAsync(host.data() + filled, src, chunk * sizeof(double), hipMemcpyDeviceToHost, h->stream));
             HIPCHK(hipStreamSynchronize(h->stream));
             ++launches;
             if (std::abs(since - h->host_dump) < 1e-6) {
-                csv_rows(h, host, hct);
+                csv_rows(h, host, hct, dct);
                 since = 0.;
                 launches = 0;
             }
@@ -900,7 +924,7 @@ int lfdm_solve(lfdm_solver* h) {
     HIPCHK(hipStreamSynchronize(h->stream));
     h->t_now = t_cont;
     if (h->output) {
-        if (launches != 0) csv_rows(h, host, launches * dct);
+        if (launches != 0) csv_rows(h, host, launches * dct, dct);
         if (int rc = csv_flush(h)) return rc;
     }
     return LFDM_OK;

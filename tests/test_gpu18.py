@@ -137,6 +137,37 @@ def test_heterogeneous_construct_groups(golden18):
         lo += n
 
 
+def test_sampled_output_matches_full_output(tmp_path):
+    """SetOutput(file, freq, store_all=false, no_outs) writes only the sampled vehicles, whose columns
+    the device gathers before the copy to the host: their CSV files are identical, byte for byte, to
+    the same vehicles' files of a store_all run.  Two Construct groups (so sampled vehicles differ),
+    2 s launches, host dumps every 4 s (mid-run and final flushes)."""
+    say("[gpu18] sampled vs full CSV output")
+    d = os.path.join(DATA, "18dof")
+    names = {}
+    for tag, store_all, no_outs in (("full", True, 0), ("some", False, 20)):
+        s = SolverHalfImplicitGPU(18, 100)
+        s.Construct(f"{d}/HMMWV/vehicle.json", f"{d}/HMMWV/tmeasy.json", 60, os.path.join(DATA, "input", "acc3.txt"))
+        s.Construct(f"{d}/HMMWV/vehicle.json", f"{d}/HMMWV/tmeasy.json", 40,
+                    os.path.join(DATA, "input", "double_lane4.txt"))
+        s.Initialize(s.zero_record(), 100)
+        s.SetEndTime(10.0)
+        s.SetKernelSimTime(2.0)
+        s.SetHostDumpTime(4.0)
+        s.SetOutput(str(tmp_path / tag), 20, store_all, no_outs)
+        s.Solve()
+        s.close()
+        names[tag] = sorted(f for f in os.listdir(tmp_path) if f.startswith(tag + "_"))
+    assert len(names["full"]) == 100 and len(names["some"]) == 20
+    sampled = [int(f[len("some_"):-len(".csv")]) for f in names["some"]]
+    assert min(sampled) < 60 <= max(sampled)  # both groups are sampled
+    for v in sampled:
+        a = open(tmp_path / f"some_{v}.csv").read()
+        b = open(tmp_path / f"full_{v}.csv").read()
+        assert a == b, v
+        assert len(a.splitlines()) == 2 + 200
+
+
 def test_ramp_sine_1024_vs_oracle(golden18):
     """The 1,024-vehicle ramp_sine workload (10 s) against the oracle at 1e-9."""
     import lfdm_oracle as O
