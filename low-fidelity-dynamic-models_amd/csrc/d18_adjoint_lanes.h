@@ -132,18 +132,23 @@ LFDM_HD void adj_forward_lane(const AdjLaunch& L, int slot, const PSet& P) {
     const int user = L.slot_user[slot];
     const int uv = user >= 0 ? user : 0;
     const int jt = L.fwd_tape_lo * L.K < L.nsteps ? L.fwd_tape_lo * L.K : L.nsteps;
+    int seg = 0, jj = 0;  // segment of step j and the step's index in it
     for (int j = 0; j < jt; ++j) {
-        if (j % L.K == 0) pack(s, L.ckpt + (size_t)(j / L.K) * kCarried * S + slot, S);
+        if (jj == 0) pack(s, L.ckpt + (size_t)seg * kCarried * S + slot, S);
         const double* c = L.ctrl + (size_t)j * 3 * N + uv;
         step18<false, false>(P, P.d, k, s, c[0], c[N], c[2 * N], L.dt, nullptr);
         if (user >= 0) {
             L.xy[(size_t)j * 2 * N + user] = s.x;
             L.xy[((size_t)j * 2 + 1) * N + user] = s.y;
         }
+        if (++jj == L.K) {
+            jj = 0;
+            ++seg;
+        }
     }
     for (int j = jt; j < L.nsteps; ++j) {
-        if (j % L.K == 0) pack(s, L.ckpt + (size_t)(j / L.K) * kCarried * S + slot, S);
-        double* row = tape_row(L, j / L.K - L.fwd_tape_lo, j % L.K, slot);
+        if (jj == 0) pack(s, L.ckpt + (size_t)seg * kCarried * S + slot, S);
+        double* row = tape_row(L, seg - L.fwd_tape_lo, jj, slot);
         tape_put_state(s, row, kWave);
         const double* c = L.ctrl + (size_t)j * 3 * N + uv;
         TireOut18 to;
@@ -152,6 +157,10 @@ LFDM_HD void adj_forward_lane(const AdjLaunch& L, int slot, const PSet& P) {
         if (user >= 0) {
             L.xy[(size_t)j * 2 * N + user] = s.x;
             L.xy[((size_t)j * 2 + 1) * N + user] = s.y;
+        }
+        if (++jj == L.K) {
+            jj = 0;
+            ++seg;
         }
     }
 }

@@ -54,9 +54,9 @@ __device__ __forceinline__ Ctl controls_at(const Launch& L, const PSet& P, CtrlS
         }
     } else {  // kCtrlKnots
         const size_t N = (size_t)L.n_user;
-        int ks = (int)floor(t / L.ds);
+        int ks = (int)floor(qdiv(t, L.ds));
         ks = ks < 0 ? 0 : (ks > L.ns - 2 ? L.ns - 2 : ks);
-        int kp = (int)floor(t / L.dp);
+        int kp = (int)floor(qdiv(t, L.dp));
         kp = kp < 0 ? 0 : (kp > L.np - 1 ? L.np - 1 : kp);
         const int key = ks * 4096 + kp;  // wave-uniform: t is the same on every lane
         if (key != cs.kcur) {
@@ -67,7 +67,7 @@ __device__ __forceinline__ Ctl controls_at(const Launch& L, const PSet& P, CtrlS
             cs.brk = L.knots[(size_t)(L.ns + L.np + kp) * N + uv];
         }
         const double tk = ks * L.ds;
-        double tbar = (t - tk) / L.ds;
+        double tbar = qdiv(t - tk, L.ds);
         if (tbar > 1.0) tbar = 1.0;
         c.s = cs.s0 + tbar * (cs.s1 - cs.s0);
         c.th = cs.thr;
