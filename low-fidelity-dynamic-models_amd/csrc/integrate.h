@@ -157,7 +157,7 @@ struct M11 {
     using X = X11;
     using K = K11;
     static constexpr int kCh = 9;
-    static constexpr int kWaves = 2;
+    static constexpr int kWaves = 3;
     template <bool NR>
     struct R {
         static constexpr int kTire = NR ? 11 : 15, kVeh = 2 * kTire, om = NR ? 0 : 4;
@@ -326,13 +326,14 @@ struct M24 {
     }
 };
 
-// Occupancy of the integration kernels (waves per SIMD).  Each model has a default, M::kWaves: the
-// 11- and 18-DOF steps need more registers than two waves per SIMD leave (the compiler spills some
-// to scratch), yet on large fleets two resident waves hide more FP64 latency than the spills cost;
-// the 24-DOF step would spill over a kilobyte per lane at two waves and runs 2.6x faster at one.
-// A fleet that fills at most one wavefront per SIMD gains nothing from the second wave's registers
-// budget, so the 11- and 18-DOF kernels also exist at one wave per SIMD (no spills); Launch::waves
-// picks (0: the model's default).
+// Occupancy of the integration kernels (waves per SIMD).  Each model has a default, M::kWaves, the
+// most waves per SIMD it is built for: on large fleets more resident waves hide more FP64 latency
+// than the register spills of a smaller budget cost, up to three for the 11-DOF step (228 B of
+// spills, +17 % over two) and two for the 18-DOF step (140 B; three would spill 520 B and run 4 %
+// slower); the 24-DOF step would spill over a kilobyte per lane at two waves and runs 2.6x faster
+// at one.  A fleet that fills at most one wavefront per SIMD gains nothing from the other waves'
+// register budget, so every kernel also exists at one wave per SIMD, and the 11-DOF one at two;
+// Launch::waves picks (0: the model's default).
 template <class M, bool NR, int MODE, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) void integrate(Launch L) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -423,8 +424,12 @@ hipError_t launch_tire(const Launch& L, int block, hipStream_t stream) {
 
 template <class M>
 hipError_t launch_model(const Launch& L, int block, hipStream_t stream) {
-    if constexpr (M::kWaves == 2) {
-        if (L.waves != 1) return launch_tire<M, 2>(L, block, stream);
+    const int w = L.waves == 0 ? M::kWaves : L.waves;
+    if constexpr (M::kWaves >= 3) {
+        if (w == 3) return launch_tire<M, 3>(L, block, stream);
+    }
+    if constexpr (M::kWaves >= 2) {
+        if (w == 2) return launch_tire<M, 2>(L, block, stream);
     }
     return launch_tire<M, 1>(L, block, stream);
 }

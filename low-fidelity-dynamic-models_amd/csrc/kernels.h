@@ -42,7 +42,7 @@ struct Launch {
     const int* horizon;
     const int* wave_hmax;
     int step_base;
-    int waves;  // waves per SIMD of the kernel: 1, 2, or 0 for the model's default
+    int waves;  // waves per SIMD of the kernel: 1 up to the model's maximum, or 0 for its default
 };
 
 // Integration launches of the 11-, 18- and 24-DOF models (one kernel per tire model and control

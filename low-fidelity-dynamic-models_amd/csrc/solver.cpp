@@ -1295,7 +1295,7 @@ int fleet_check(lfdm_fleet* f) {
 }
 
 // Occupancies the model's integration kernels are built for (24-DOF: one wave per SIMD only).
-int max_waves(int dof) { return dof == 24 ? 1 : 2; }
+int max_waves(int dof) { return dof == 24 ? 1 : (dof == 18 ? 2 : 3); }
 
 void fleet_drop_graph(lfdm_fleet* f) {
     if (f->exec) (void)hipGraphExecDestroy(f->exec);
@@ -1411,9 +1411,9 @@ int lfdm_fleet_set_config(lfdm_fleet* f, int bucket, int block, int steps_per_la
     if (block < 64 || block > 256 || block % kWave) return fail(LFDM_E_ARG, "block size must be 64, 128, 192 or 256");
     if (steps_per_launch < 0) return fail(LFDM_E_ARG, "steps per launch must be >= 0");
     if (waves_per_simd < 0 || waves_per_simd > max_waves(f->buckets[bucket]->dof))
-        return fail(LFDM_E_ARG, "waves per SIMD must be 0 (automatic) or 1" +
-                                    std::string(max_waves(f->buckets[bucket]->dof) > 1 ? " or 2" : "") +
-                                    " for the " + std::to_string(f->buckets[bucket]->dof) + "-DOF kernel");
+        return fail(LFDM_E_ARG, "waves per SIMD must be 0 (automatic) or 1 to " +
+                                    std::to_string(max_waves(f->buckets[bucket]->dof)) + " for the " +
+                                    std::to_string(f->buckets[bucket]->dof) + "-DOF kernel");
     f->cfg[bucket].block = block;
     f->cfg[bucket].spl = steps_per_launch;
     f->cfg[bucket].waves = waves_per_simd;
