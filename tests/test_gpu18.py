@@ -235,6 +235,42 @@ def test_knot_fleet_vs_oracle_and_batch_independence(golden18):
     s3.close()
 
 
+def test_knot_fleet_tmeasy_nr_vs_oracle(golden18):
+    """The TMeasyNr kernels at fleet scale: 2,048 Sedan vehicles with the reference's TMeasyNr tire
+    (no relaxation states, Coulomb blend at low speed), knot_fleet controls and jittered parameters,
+    4 s, against the oracle, with the ulp-stability bar of the TMeasy fleet test."""
+    import lfdm_oracle as O
+    from cases18 import ULP_FIELDS
+
+    say("[gpu18] knot_fleet TMeasyNr 2048 vehicles vs oracle")
+    flat = golden18["sedan_nr_acc05/params"]
+    n, nsteps, every = 2048, 4000, 200
+    fl = W.knot_fleet(n, flat, seed=9, tire_nr=True)
+    ctrl = ("knots", fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
+    s = SolverHalfImplicitGPU(18, n)
+    s.construct_flat(flat, n, None, TMeasyNr)
+    s.Initialize(s.zero_record(), n)
+    s.set_overrides(fl["ovr"])
+    s.set_controls_knots(fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
+    gpu = run_sampled(s, nsteps, every)
+    s.close()
+    _, ref = O.run18(flat, ctrl, n, nsteps, tire_nr=True, ovr=fl["ovr"], every=every)
+    err = normwise_rel(gpu, ref).max(axis=-1)
+    spread = np.zeros(n)
+    for f in ULP_FIELDS:
+        for direction in (np.inf, -np.inf):
+            o2 = fl["ovr"].copy()
+            o2[f] = np.nextafter(o2[f], direction)
+            _, tr = O.run18(flat, ctrl, n, nsteps, tire_nr=True, ovr=o2, every=every)
+            spread = np.maximum(spread, normwise_rel(tr, ref).max(axis=-1))
+    stable = spread <= 1e-11
+    say(f"[gpu18] knot_fleet TMeasyNr: {stable.sum()} ulp-stable vehicles, max rel {err[stable].max():.3e}; "
+        f"{(~stable).sum()} ill-conditioned (spread up to {spread.max():.2e}), {(err[~stable] <= TOL).sum()} of "
+        f"them within 1e-9; bitwise fraction {np.mean(gpu == ref):.4f}")
+    assert stable.mean() > 0.8
+    assert err[stable].max() <= TOL
+
+
 def test_chunking_and_block_size_invariance(golden18):
     """Results do not depend on kernel_sim_time (launch chunking) or threads per block."""
     say("[gpu18] chunking invariance")
