@@ -171,6 +171,19 @@ int lfdm_adjoint_backward(lfdm_solver* h, const double* device_xy_bar, double* d
  * a group holds as many segments as fill the GPU, within this budget, and at least one.  Results do
  * not depend on it. */
 int lfdm_adjoint_set_tape_budget(lfdm_solver* h, unsigned long long bytes);
+/* Exact Jacobian of one step at every vehicle's current record, for linearizing the model (e.g. MPC).
+ * Replaces d18SolverHalfImplicit::IntegrateStepWithJacobian + GetJacobianState / GetJacobianControls
+ * (18dof/dof18_halfImplicit.h:79-86, 159; dof18_halfImplicit.cpp:356-463), which differences the
+ * right-hand side for one vehicle on the CPU (relative step 1e-3, at gear 0 with zero lagged
+ * accelerations).  Here the step adjoint gives the discrete step's exact derivatives at the record as
+ * it is, for every vehicle at once: device_jac[21][24][total], rows the stepped packed states, columns
+ * the packed states then steering, throttle, braking, both in the reference's packY order of a vehicle
+ * with a torque converter (xe ye per tire, omega x 4, crankOmega, x y u v psi wz phi wx).  Without a
+ * converter crankOmega is no state: its column is zero and its row the derivative of the slaved
+ * crank speed.  device_ctrl[3][total]: the step's steering, throttle, braking.  The records do not
+ * change (advance the step separately).  *rows / *cols receive 21 / 24.  18-DOF TMeasy only, like the
+ * adjoint; asynchronous on the handle's stream. */
+int lfdm_step_jacobian(lfdm_solver* h, const double* device_ctrl, double* device_jac, int* rows, int* cols);
 
 /* ---- batching form: per-vehicle horizons, layouts, mixed fleets (no counterpart in the
  * reference, whose solver has one end time for every vehicle and overshoots it to a whole number of

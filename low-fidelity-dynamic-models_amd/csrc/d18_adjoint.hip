@@ -6,6 +6,8 @@
 //    the wavefronts of the fleet and the recompute runs at full occupancy even for fleets that fill
 //    only one wavefront per SIMD.  The tape is blocked by wavefront (one contiguous 19 KB run per
 //    wavefront and step) and the per-vehicle constants sit in LDS.
+//  - step_jacobian: one lane per slot; the step's Jacobian at the current record, one step adjoint
+//    sweep per row.
 //  - adj_reverse: one lane per slot; walks the group's segments backwards through the step adjoint
 //    of d18_adjoint.h.  The step adjoint needs the whole register file of a SIMD, so the per-vehicle
 //    constants and their adjoints live in LDS (23 + 21 doubles per lane; both structs have an odd
@@ -61,7 +63,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1))
     adj_reverse_lane(L, slot, pset_at(L.sets, uniform(L.wave_set[wave])), ks[threadIdx.x], kbs[threadIdx.x]);
 }
 
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void step_jacobian(AdjLaunch L) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const int wave = uniform(slot >> 6);
+    if (wave * kWave >= L.S) return;
+    jac_lane(L, slot, pset_at(L.sets, uniform(L.wave_set[wave])));
+}
+
 }  // namespace
+
+hipError_t launch_step_jacobian(const AdjLaunch& L, hipStream_t stream) {
+    if (L.S <= 0) return hipSuccess;
+    hipLaunchKernelGGL(step_jacobian, (L.S + kThreads - 1) / kThreads, kThreads, 0, stream, L);
+    return hipGetLastError();
+}
 
 hipError_t launch_adj_forward(const AdjLaunch& L, int waves_per_simd, hipStream_t stream) {
     if (L.S <= 0 || L.nsteps <= 0) return hipSuccess;

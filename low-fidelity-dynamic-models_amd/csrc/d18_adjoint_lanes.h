@@ -299,5 +299,52 @@ LFDM_HD void adj_reverse_lane(const AdjLaunch& L, int slot, const PSet& P, K18& 
     for (int i = 0; i < 7; ++i) L.param_bar[i * N + user] = pb[i];
 }
 
+// Step Jacobian of one slot at its current record: one adjoint sweep of the step per stepped packed
+// state (rows), giving its derivatives with respect to the packed states and the three controls
+// (columns).  The packed order is the reference's packY order of a vehicle with a torque converter
+// (dof18_halfImplicit.cpp:320-341); without a converter crankOmega is no state of the step (the
+// reference leaves it out): its column is zero and its row the derivative of the slaved crank speed.
+// The lagged accelerations, deflection rates and gear of the record are held at their values.
+LFDM_HD void jac_lane(const AdjLaunch& L, int slot, const PSet& P) {
+    const int user = L.slot_user[slot];
+    if (user < 0) return;
+    const size_t S = (size_t)L.S, N = (size_t)L.n_user;
+    S18 s;
+    load_record(L.state, S, slot, s);
+    K18 k;
+    load_k(P, L.ovr + slot, S, k);
+    const double steer = L.ctrl[user], thr = L.ctrl[N + user], brk = L.ctrl[2 * N + user];
+    for (int r = 0; r < kJacRows; ++r) {
+        S18b sb;
+        s18b_zero(sb);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            sb.xe[i] = r == 2 * i ? 1.0 : 0.0;
+            sb.ye[i] = r == 2 * i + 1 ? 1.0 : 0.0;
+            sb.omega[i] = r == 8 + i ? 1.0 : 0.0;
+        }
+        sb.crank = r == 12 ? 1.0 : 0.0;
+        sb.x = r == 13 ? 1.0 : 0.0;
+        sb.y = r == 14 ? 1.0 : 0.0;
+        sb.u = r == 15 ? 1.0 : 0.0;
+        sb.v = r == 16 ? 1.0 : 0.0;
+        sb.psi = r == 17 ? 1.0 : 0.0;
+        sb.wz = r == 18 ? 1.0 : 0.0;
+        sb.phi = r == 19 ? 1.0 : 0.0;
+        sb.wx = r == 20 ? 1.0 : 0.0;
+        K18b kb;
+        k18b_zero(kb);
+        double gs = 0, gt = 0, gb = 0;
+        step18_adj(P, P.d, k, s, steer, thr, brk, L.dt, sb, gs, gt, gb, kb);
+        const double col[kJacCols] = {sb.xe[0], sb.ye[0], sb.xe[1], sb.ye[1], sb.xe[2], sb.ye[2], sb.xe[3], sb.ye[3],
+                                      sb.omega[0], sb.omega[1], sb.omega[2], sb.omega[3],
+                                      P.tcbool ? sb.crank : 0.0, sb.x, sb.y, sb.u, sb.v, sb.psi, sb.wz, sb.phi, sb.wx,
+                                      gs, gt, gb};
+        double* row = L.jac + (size_t)r * kJacCols * N + user;
+#pragma unroll
+        for (int j = 0; j < kJacCols; ++j) row[(size_t)j * N] = col[j];
+    }
+}
+
 }  // namespace dev
 }  // namespace lfdm

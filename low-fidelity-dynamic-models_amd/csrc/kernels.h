@@ -83,13 +83,22 @@ struct AdjLaunch {
     int nsteps, K, S, n_user;
     int seg_lo, seg_hi;  // backward: the group of this launch
     int fwd_tape_lo;     // forward: segments from this one on also go to the tape (block 0 onwards)
+    double* jac;         // step Jacobian: [kJacRows][kJacCols][n_user]
 };
+
+// Step Jacobian (lfdm_step_jacobian): rows are the stepped packed states, columns the packed states
+// then steering, throttle and braking, both in the reference's packY order of a vehicle with a torque
+// converter (xe ye per tire, omega x 4, crankOmega, x y u v psi wz phi wx).
+constexpr int kJacRows = 21;
+constexpr int kJacCols = kJacRows + 3;
 
 // waves_per_simd 1 or 2: register budget of the forward kernel (1 when the fleet fills at most one
 // wavefront per SIMD, so the step runs without spills).
 hipError_t launch_adj_forward(const AdjLaunch& L, int waves_per_simd, hipStream_t stream);
 hipError_t launch_adj_recompute(const AdjLaunch& L, hipStream_t stream);
 hipError_t launch_adj_reverse(const AdjLaunch& L, hipStream_t stream);
+// Every vehicle's step Jacobian at its current record under the per-vehicle controls ctrl[3][n_user].
+hipError_t launch_step_jacobian(const AdjLaunch& L, hipStream_t stream);
 
 // Gathers full records of `n` slots from SoA state into out[n][rec].
 hipError_t launch_gather(const double* state, int S, int rec, const int* slots, int n, double* out,

@@ -1225,6 +1225,23 @@ int lfdm_adjoint_set_tape_budget(lfdm_solver* h, unsigned long long bytes) {
     return LFDM_OK;
 }
 
+int lfdm_step_jacobian(lfdm_solver* h, const double* dctrl, double* djac, int* rows, int* cols) {
+    if (int rc = check(h)) return rc;
+    if (!dctrl || !djac) return fail(LFDM_E_ARG, "null controls or Jacobian buffer");
+    if (int rc = adjoint_supported(h)) return rc;
+    if (int rc = ensure_device(h)) return rc;
+    if (int rc = sync_inputs(h)) return rc;
+    AdjLaunch L = adj_launch(h);
+    L.ctrl = dctrl;
+    L.jac = djac;
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    HIPCHK(launch_step_jacobian(L, h->stream));
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    if (rows) *rows = kJacRows;
+    if (cols) *cols = kJacCols;
+    return LFDM_OK;
+}
+
 int lfdm_adjoint_backward(lfdm_solver* h, const double* dxy_bar, double* dctrl_bar, double* dstate_bar,
                           double* dparam_bar) {
     if (int rc = check(h)) return rc;

@@ -269,6 +269,14 @@ class SolverHalfImplicitGPU:
     def adjoint_set_tape_budget(self, nbytes):
         check(lib.lfdm_adjoint_set_tape_budget(self._h, int(nbytes)))
 
+    def step_jacobian(self, ctrl_ptr, jac_ptr):
+        """Exact Jacobian of one step at every vehicle's current record (device pointers: controls
+        [3][total], Jacobian [21][24][total]; see lfdm_step_jacobian).  Returns (rows, cols)."""
+        r, c = ctypes.c_int(), ctypes.c_int()
+        check(lib.lfdm_step_jacobian(self._h, ctypes.c_void_p(ctrl_ptr), ctypes.c_void_p(jac_ptr), ctypes.byref(r),
+                                     ctypes.byref(c)))
+        return r.value, c.value
+
     def set_stream(self, stream_ptr):
         check(lib.lfdm_set_stream(self._h, ctypes.c_void_p(stream_ptr)))
 
@@ -314,9 +322,6 @@ class Fleet:
 
     def __exit__(self, *exc):
         self.close()
-
-    def adjoint_set_tape_budget(self, nbytes):
-        check(lib.lfdm_adjoint_set_tape_budget(self._h, int(nbytes)))
 
     def set_stream(self, stream_ptr):
         check(lib.lfdm_fleet_set_stream(self._f, ctypes.c_void_p(stream_ptr)))

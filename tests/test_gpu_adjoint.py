@@ -212,3 +212,59 @@ def test_default_stream_orders_with_torch():
     assert torch.isfinite(xy).all()
     assert torch.equal(early, xy)
     s.close()
+
+
+@pytest.mark.parametrize("veh,tire", [("HMMWV/vehicle.json", "HMMWV/tmeasy.json"),
+                                      ("audi/vehicle.json", "audi/tmeasy.json")])
+def test_step_jacobian_matches_host_adjoint(veh, tire):
+    """lfdm_step_jacobian (one step adjoint sweep per row on the GPU) against the same step adjoint
+    compiled for the host (tests/cpp/jacobian_check.cpp, mode "asis"), vehicle by vehicle, at records
+    reached by 3 s of knot-controlled driving and under per-vehicle random controls: row-wise
+    normwise relative difference <= 1e-9 (host and device libm may differ in the last bit); the
+    records do not change; without a torque converter (audi) the crankOmega column is zero.  The
+    host step adjoint itself is checked against the reference's IntegrateStepWithJacobian and
+    against finite differences (tests/test_adjoint_host.py)."""
+    import os
+    import subprocess
+
+    from conftest import ROOT
+
+    say(f"[adjoint] step Jacobian {veh}")
+    d = f"{DATA}/18dof"
+    flat = lfdm_amd.params_from_json(18, f"{d}/{veh}", f"{d}/{tire}")
+    n = 300
+    fl = W.knot_fleet(n, flat, seed=6, jitter=0.0)
+    s = lfdm_amd.SolverHalfImplicitGPU(18, n)
+    s.construct_flat(flat, n, None, lfdm_amd.TMeasy)
+    s.Initialize(s.zero_record(), n)
+    s.set_overrides(fl["ovr"])
+    s.set_controls_knots(fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
+    s.advance(3000)
+    rec = s.get_states()
+    rng = np.random.default_rng(11)
+    ctrl = np.stack([rng.uniform(-0.4, 0.4, n), rng.uniform(0, 1, n), rng.uniform(0, 0.3, n)])
+    dc = torch.tensor(ctrl, dtype=torch.float64, device="cuda")
+    dj = torch.full((21, 24, n), float("nan"), dtype=torch.float64, device="cuda")
+    rows, cols = s.step_jacobian(dc.data_ptr(), dj.data_ptr())
+    s.synchronize()
+    assert (rows, cols) == (21, 24)
+    jac = dj.cpu().numpy()
+    np.testing.assert_array_equal(s.get_states(), rec)
+    s.close()
+    assert np.isfinite(jac).all()
+    tool = os.path.join(ROOT, "low-fidelity-dynamic-models_amd", "lfdm_amd", "jacobian_check")
+    worst = 0.0
+    for v in (0, 37, 150, 299):
+        inp = " ".join(repr(float(x)) for x in (*ctrl[:, v], 1e-3, *rec[v]))
+        r = subprocess.run([tool, f"{d}/{veh}", f"{d}/{tire}", "asis"], input=inp, capture_output=True, text=True,
+                           timeout=60)
+        assert r.returncode == 0, r.stderr
+        host = np.array([[float(x) for x in line.split()] for line in r.stdout.splitlines()])
+        assert host.shape == (21, 24)
+        g = jac[:, :, v]
+        err = np.abs(g - host).max(axis=1) / np.maximum(np.abs(host).max(axis=1), 1e-300)
+        worst = max(worst, err.max())
+    say(f"[adjoint] step Jacobian {veh}: worst row vs host {worst:.2e}")
+    assert worst <= 1e-9
+    if "audi" in veh:
+        assert np.all(jac[:, 12, :] == 0.0)
