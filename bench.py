@@ -448,7 +448,7 @@ def main():
         port = {"value": rate, "unit": "vehicle-steps/s", "cores": threads, "kind": "port",
                 "sample": f"knot_fleet, {args.cpu_sample} vehicles x {H} steps ({secs:.1f} s), "
                           f"oracle (bit-identical CPU restatement), OpenMP over vehicles",
-                "single_core": rate1, "cpu": cpu_model()}
+                "single_core": rate1, "per_core": rate / threads, "cpu": cpu_model(), "nproc": os.cpu_count()}
         ref = cpu_reference(args.ref_sample, H, threads) if dof == 18 else None
         if ref is None:
             cpu = port
@@ -457,7 +457,8 @@ def main():
                    "sample": f"knot_fleet, {args.ref_sample} vehicles x {H} steps ({ref[2]:.1f} s), the reference's "
                              f"own CPU solver (d18SolverHalfImplicit::IntegrateStep compiled from its sources into "
                              f"oracle/_ref), one solver object per vehicle, {threads} threads",
-                   "single_core": ref[1], "cpu": cpu_model(), "port": port}
+                   "single_core": ref[1], "per_core": ref[0] / threads, "cpu": cpu_model(),
+                   "nproc": os.cpu_count(), "port": port}
     metric = ("18-DOF FP64 vehicle-steps/s at 1M vehicles; adjoint/forward cost ratio" if dof == 18 else
               f"{dof}-DOF FP64 vehicle-steps/s")
     other = None
