@@ -414,7 +414,7 @@ def main():
         print(json.dumps({"copy_bytes": a.numel() * 8, "launches": 3}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; vehicles shard across GPUs with no "
+        print(json.dumps({"error": "bench.py measures one GPU only; vehicles shard across GPUs with no "
                                    "communication between them, run with --gpus 1"}))
         sys.exit(2)
 
