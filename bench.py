@@ -354,17 +354,23 @@ def mixed_fleet(n_total=2_000_000, split=(0.4, 0.4, 0.2), tmin=2.0, tmax=20.0, j
             "speedup_vs_grouped_eager": eager_wall / wall}
 
 
-def measured_traffic_per_step():
-    """HBM bytes per vehicle-step of the 18-DOF integration kernel from the calibrated FETCH_SIZE +
-    WRITE_SIZE counter runs summarized in profiles/ (None if the summary is absent)."""
+def counters_figure(key):
+    """A figure of the 18-DOF integration kernel's counter runs summarized in profiles/ (None if the
+    summary is absent): "traffic_bytes_per_vehicle_step" (calibrated FETCH_SIZE + WRITE_SIZE) or
+    "valu_busy_per_simd" (SQ_ACTIVE_INST_VALU / SQ_WAVE_CYCLES x waves per SIMD)."""
     path = os.path.join(ROOT, "profiles", "integrate_18dof_knot_fleet_counters.md")
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_vehicle_step:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
+
+
+def measured_traffic_per_step():
+    """HBM bytes per vehicle-step of the 18-DOF integration kernel (None if unmeasured)."""
+    return counters_figure("traffic_bytes_per_vehicle_step")
 
 
 def cpu_model():
@@ -440,7 +446,10 @@ def main():
                 "traffic_source": "profiles/integrate_18dof_knot_fleet_counters.md (bytes per vehicle-step x "
                                   "vehicle-steps of one launch)" if tps is not None else None,
                 "hbm_gbs": None if tps is None else tps * n * H / (kernel_ms * 1e-3) / 1e9,
-                "hbm_frac": None if tps is None else tps * n * H / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                "hbm_frac": None if tps is None else tps * n * H / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "valu_busy_per_simd": counters_figure("valu_busy_per_simd") if dof == 18 else None,
+                "valu_busy_source": "profiles/integrate_18dof_knot_fleet_counters.md (fraction of SIMD cycles "
+                                    "issuing a VALU instruction; the step's bound)" if dof == 18 else None}
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
