@@ -168,8 +168,9 @@ int lfdm_adjoint_backward(lfdm_solver* h, const double* device_xy_bar, double* d
 /* Device memory the adjoint may use for its tape (default 16 GiB), from the next
  * lfdm_adjoint_forward on.  The backward walks the checkpoint segments in groups, recomputing a whole
  * group's steps at once (38 doubles per vehicle and step; the forward tapes the last group itself);
- * a group holds as many segments as fill the GPU, within this budget, and at least one.  Results do
- * not depend on it. */
+ * a group holds as many segments as this budget does, at least one.  A budget that holds the whole
+ * horizon (steps x 38 x total x 8 bytes) makes the forward tape every step and the backward recompute
+ * nothing.  Results do not depend on it. */
 int lfdm_adjoint_set_tape_budget(lfdm_solver* h, unsigned long long bytes);
 /* Exact Jacobian of one step at every vehicle's current record, for linearizing the model (e.g. MPC).
  * Replaces d18SolverHalfImplicit::IntegrateStepWithJacobian + GetJacobianState / GetJacobianControls

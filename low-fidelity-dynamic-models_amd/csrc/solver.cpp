@@ -1170,16 +1170,12 @@ AdjLaunch adj_launch(lfdm_solver* h) {
 }
 
 
-// Segments per backward group: enough (segment, vehicle) lanes for about four wavefronts per SIMD
-// in the recompute launch, within the handle's tape budget (at least one segment).
+// Segments per backward group: as many as the handle's tape budget holds, at least one and at most
+// all.  The forward tapes the group the backward walks first, so every segment beyond the first group
+// costs one recompute; with the whole horizon in the budget the backward recomputes nothing.
 int adj_group(lfdm_solver* h, int nseg) {
-    const int simds = simd_count(h);
-    if (simds < 0) return simds;
-    const size_t waves = (size_t)h->S / kWave;
-    const size_t want = 4 * (size_t)simds;
-    size_t G = (want + waves - 1) / std::max<size_t>(waves, 1);
     const size_t per_seg = (size_t)h->adj_K * kAdjTapeRow * (size_t)h->S * sizeof(double);
-    G = std::min(G, std::max<size_t>(h->adj_tape_budget / std::max<size_t>(per_seg, 1), 1));
+    const size_t G = h->adj_tape_budget / std::max<size_t>(per_seg, 1);
     return (int)std::max<size_t>(1, std::min<size_t>(G, (size_t)nseg));
 }
 
@@ -1202,7 +1198,6 @@ int lfdm_adjoint_forward(lfdm_solver* h, int nsteps, int checkpoint_every, const
     h->tape_holds_last = false;
     // the forward also tapes the group the backward walks first
     const int G = adj_group(h, (int)nseg);
-    if (G < 0) return G;
     h->adj_G = G;
     if (int rc = ensure_buffer(h->d_tape, h->tape_cap, (size_t)G * K * kAdjTapeRow * (size_t)h->S)) return rc;
     AdjLaunch L = adj_launch(h);

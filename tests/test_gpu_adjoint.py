@@ -176,13 +176,16 @@ def test_torch_rollout_backward_and_mpc_step():
     s.adjoint_set_tape_budget(1)
     xy = rollout(s, ctrl, p, i0, 100)
     tracking_loss(xy, ref).sum().backward()
-    s.adjoint_set_tape_budget(16 << 30)
     assert torch.equal(ctrl.grad, g1)
     gp1, gi1 = p.grad.clone(), i0.grad.clone()
-    ctrl.grad = p.grad = i0.grad = None
-    xy = rollout(s, ctrl, p, i0, 100)
-    tracking_loss(xy, ref).sum().backward()
-    assert torch.equal(p.grad, gp1) and torch.equal(i0.grad, gi1)
+    # groups of 3 of the 15 segments (the forward tapes the last group, the backward recomputes the
+    # other four), then the default budget, which holds all 15: the forward tapes every step
+    for budget in (3 * 100 * 38 * n * 8, 16 << 30):
+        ctrl.grad = p.grad = i0.grad = None
+        s.adjoint_set_tape_budget(budget)
+        xy = rollout(s, ctrl, p, i0, 100)
+        tracking_loss(xy, ref).sum().backward()
+        assert torch.equal(ctrl.grad, g1) and torch.equal(p.grad, gp1) and torch.equal(i0.grad, gi1)
     say("[adjoint] MPC, 6 iterations")
     _, hist = run_mpc(s, flat, init, params, ref, c0, iterations=6, lr=0.05)
     say(f"[adjoint] MPC mean loss per vehicle: {[round(h, 4) for h in hist]}")
