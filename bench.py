@@ -285,25 +285,18 @@ def mixed_fleet(n_total=2_000_000, split=(0.4, 0.4, 0.2), tmin=2.0, tmax=20.0, j
     from lfdm_amd import LAYOUT_GROUPED, Fleet, SolverHalfImplicitGPU, TMeasy
     from lfdm_amd import workloads as W
 
-    rng = np.random.Generator(np.random.PCG64(seed))
-    dt = 1e-3
     buckets, inits, horizons = [], [], []
-    for k, (dof, frac) in enumerate(zip((11, 18, 24), split)):
-        n = int(round(n_total * frac))
-        d = os.path.join(W.DATA, f"{dof}dof", "HMMWV")
-        flat = lfdm_amd.params_from_json(dof, f"{d}/vehicle.json", f"{d}/tmeasy.json",
-                                         f"{d}/suspension.json" if dof == 24 else None)
-        fl = W.knot_fleet(n, flat, seed=seed + k, horizon=tmax, jitter=jitter, dof=dof)
+    for m in W.mixed_fleet(n_total, split, tmin, tmax, jitter, seed):
+        dof, n, flat, fl, hz = m["dof"], m["n"], m["flat"], m["fl"], m["horizon"]
         s = SolverHalfImplicitGPU(dof, n)
         s.construct_flat(flat, n, None, TMeasy)
         s.set_overrides(fl["ovr"])
         if dof == 24:
-            s.initialize_batch(W.initial_speed_records(n, flat, dof, seed=seed + k))
+            s.initialize_batch(m["recs"])
         else:
             s.Initialize(s.zero_record(), n)
         s.set_controls_knots(fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
-        del fl
-        hz = np.rint(rng.uniform(tmin, tmax, n) / dt).astype(np.int32)
+        del fl, m
         s.set_horizons(hz)
         buckets.append(s)
         inits.append(s.get_states())

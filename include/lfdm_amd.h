@@ -142,6 +142,19 @@ int lfdm_chunk_steps(lfdm_solver* h, double t, int* nsteps);
 /* Number of vehicles initialized so far (Initialize / initialize_batch / set_states). */
 int lfdm_initialized_count(lfdm_solver* h, unsigned* count);
 
+/* Arithmetic of the integration step (no counterpart in the reference, which has one form).
+ * LFDM_ARITH_FAST (the default) evaluates the reference's functions in a few other forms of the same
+ * value: the lateral slip -tan(atan2(vsy, vta) - delta) as a tangent difference, hypot(a, b) as
+ * sqrt(a a + b b), and divisions by a shared denominator as multiplications by its reciprocal
+ * (dof18.cpp:412-539, 1019-1078 and their 11/24-DOF twins).  LFDM_ARITH_STRICT evaluates every one of
+ * them as the reference writes it (atan2 and tan, hypot, one correctly rounded division per quotient),
+ * so results differ from the reference CPU solver only where the device libm and glibc round
+ * differently; it runs at the model's default occupancy and costs more instructions per step. */
+#define LFDM_ARITH_FAST 0
+#define LFDM_ARITH_STRICT 1
+int lfdm_set_arithmetic(lfdm_solver* h, int arith);
+int lfdm_get_arithmetic(lfdm_solver* h, int* arith);
+
 /* Advances every vehicle nsteps steps from its current time in one launch (no output), on the
  * handle's stream, without synchronizing.  The time base and the position in per-step controls
  * continue across calls. */

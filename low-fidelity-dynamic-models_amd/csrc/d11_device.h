@@ -47,7 +47,8 @@ LFDM_HD void k11_init(const PSet& P, double m, double jz, double dfy0Pn, double 
 }
 
 // One half-implicit step (rhsFun order, dof11_halfImplicit.cpp:560-605; update :172-235).
-template <bool NR, bool FULL>
+// STRICT: the reference's own quotients and transcendentals throughout (see dev_tire.h).
+template <bool NR, bool FULL, bool STRICT = false>
 LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, double steering,
                                        double throttle, double braking, double dt, X11* x) {
     const TireP& t = P.tire;
@@ -71,19 +72,21 @@ LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, doubl
     double fx[2], fy[2], My[2], xt[2], rStat[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
+        const double dl = i == 0 ? delta : u.delta_rear;
         const double d = i == 0 ? qdiv(sd, cd) : u.tan_delta_rear;
         if (fz[i] <= 0) {
             if (!NR) { s.xe[i] = 0; s.ye[i] = 0; }
             s.omega[i] = 0; xt[i] = 0; rStat[i] = t.r0; fx[i] = 0; fy[i] = 0; My[i] = 0;
             continue;
         }
-        xt[i] = fz[i] * u.inv_kt;
+        xt[i] = STRICT ? qdiv(fz[i], t.kt) : fz[i] * u.inv_kt;
         rStat[i] = t.r0 - xt[i];
         if (!NR)
-            tmeasy_core(t, u, k.qy1, k.qy2, d, fz[i], rStat[i], vsx[i], vsy[i], s.xe[i], s.ye[i], s.omega[i],
+            tmeasy_core<STRICT>(t, u, k.qy1, k.qy2, dl, d, fz[i], rStat[i], vsx[i], vsy[i], s.xe[i], s.ye[i], s.omega[i],
                         s.xedot[i], s.yedot[i], fx[i], fy[i], My[i]);
         else
-            tmeasy_nr_core(t, u, k.qy1, k.qy2, d, fz[i], rStat[i], vsx[i], vsy[i], s.omega[i], fx[i], fy[i], My[i]);
+            tmeasy_nr_core<STRICT>(t, u, k.qy1, k.qy2, dl, d, fz[i], rStat[i], vsx[i], vsy[i], s.omega[i], fx[i], fy[i],
+                                   My[i]);
     }
     // computePowertrainRHS (dof11.cpp:607-749)
     double torque_t;

@@ -214,7 +214,7 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     // ---- forward values (the step's own evaluation) ----
     const double rStat = t.r0 - fz * u.inv_kt;
     TireEval w;
-    tmeasy_eval(t, u, qy1, qy2, tan_delta, fz, rStat, vsx, vsy, xe, ye, omega, w);
+    tmeasy_eval(t, u, qy1, qy2, 0.0, tan_delta, fz, rStat, vsx, vsy, xe, ye, omega, w);
     const double rdynco = w.rdynco, drd = w.drd, r_eff = w.r_eff, vsx2 = w.vsx2, aom = w.aom, som = w.som;
     const double rw = w.rw, vta = w.vta, sx = w.sx, sy = w.sy, ta = -sy;  // ta: tan of the slip angle
     const bool clip = w.clip;
@@ -573,8 +573,8 @@ LFDM_HD void step18_adj(const PSet& P, const Derived& u, const K18& k, const S18
             yedot[i] = to->yedot[i];
         } else {
             double xt;
-            tire_tmeasy(t, u, k, i < 2 ? qdiv(sd, cd) : u.tan_delta_rear, fz[i], vsx[i], vsy[i], xe[i], ye[i], xedot[i],
-                        yedot[i], om[i], fx[i], fy[i], My[i], xt, rStat[i]);
+            tire_tmeasy(t, u, k, 0.0, i < 2 ? qdiv(sd, cd) : u.tan_delta_rear, fz[i], vsx[i], vsy[i], xe[i], ye[i],
+                        xedot[i], yedot[i], om[i], fx[i], fy[i], My[i], xt, rStat[i]);
         }
     }
     double crank = s0.crank;

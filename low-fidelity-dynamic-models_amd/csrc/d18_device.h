@@ -58,7 +58,7 @@ LFDM_HD void k18_init(const PSet& P, double m, double jz, double jx, double dfy0
     k.A3 = A3;
     k.inv_mt = 1 / mt;
     k.two_hrc_m = 2. * hrc * m;
-    // the chassis solve's determinant, kept as its reciprocal (each step multiplies by it)
+    // the chassis solve's determinant, kept as its reciprocal (the fast step multiplies by it)
     k.rdenom = 1. / (A2 * (A1 * A1) - 2. * A1 * A3 * P.jxz + jz * (A3 * A3) + mt * (P.jxz * P.jxz) - A2 * jz * mt);
     k.A1A2 = A1 * A2;
     k.A1A3 = A1 * A3;
@@ -69,8 +69,9 @@ LFDM_HD void k18_init(const PSet& P, double m, double jz, double jx, double dfy0
 
 // Tire RHS, TMeasy (dof18.cpp:380-539): vertical load from the load transfer, static radius from the
 // tire compression, then the force core.  Off the ground the tire resets and keeps its deflection
-// rates (dof18.cpp:399-409).
-LFDM_HD void tire_tmeasy(const TireP& t, const Derived& u, const K18& k, double tan_delta, double fz,
+// rates (dof18.cpp:399-409).  delta / tan_delta: the tire's steering angle and its tangent.
+template <bool STRICT = false>
+LFDM_HD void tire_tmeasy(const TireP& t, const Derived& u, const K18& k, double delta, double tan_delta, double fz,
                                             double vsx, double vsy, double& xe, double& ye, double& xedot,
                                             double& yedot, double& omega, double& fx, double& fy, double& My,
                                             double& xt, double& rStat) {
@@ -78,22 +79,24 @@ LFDM_HD void tire_tmeasy(const TireP& t, const Derived& u, const K18& k, double 
         xe = 0; ye = 0; omega = 0; xt = 0; rStat = t.r0; fx = 0; fy = 0; My = 0;
         return;
     }
-    xt = fz * u.inv_kt;
+    xt = STRICT ? qdiv(fz, t.kt) : fz * u.inv_kt;
     rStat = t.r0 - xt;
-    tmeasy_core(t, u, k.qy1, k.qy2, tan_delta, fz, rStat, vsx, vsy, xe, ye, omega, xedot, yedot, fx, fy, My);
+    tmeasy_core<STRICT>(t, u, k.qy1, k.qy2, delta, tan_delta, fz, rStat, vsx, vsy, xe, ye, omega, xedot, yedot, fx,
+                        fy, My);
 }
 
 // Tire RHS, TMeasyNr (dof18.cpp:541-673).
-LFDM_HD void tire_tmeasy_nr(const TireP& t, const Derived& u, const K18& k, double tan_delta, double fz,
-                                               double vsx, double vsy, double& omega, double& fx, double& fy,
-                                               double& My, double& xt, double& rStat) {
+template <bool STRICT = false>
+LFDM_HD void tire_tmeasy_nr(const TireP& t, const Derived& u, const K18& k, double delta, double tan_delta,
+                                               double fz, double vsx, double vsy, double& omega, double& fx,
+                                               double& fy, double& My, double& xt, double& rStat) {
     if (fz <= 0) {
         omega = 0; xt = 0; rStat = t.r0; fx = 0; fy = 0; My = 0;
         return;
     }
-    xt = fz * u.inv_kt;
+    xt = STRICT ? qdiv(fz, t.kt) : fz * u.inv_kt;
     rStat = t.r0 - xt;
-    tmeasy_nr_core(t, u, k.qy1, k.qy2, tan_delta, fz, rStat, vsx, vsy, omega, fx, fy, My);
+    tmeasy_nr_core<STRICT>(t, u, k.qy1, k.qy2, delta, tan_delta, fz, rStat, vsx, vsy, omega, fx, fy, My);
 }
 
 // One half-implicit step (rhsFun order, dof18_halfImplicit.cpp:618-673; update :245-318).
@@ -103,7 +106,8 @@ struct TireOut18 {
     double fx[4], fy[4], My[4], rStat[4], xedot[4], yedot[4];
 };
 
-template <bool NR, bool FULL, bool CAPTURE = false>
+// STRICT: the reference's own quotients and transcendentals throughout (see dev_tire.h).
+template <bool NR, bool FULL, bool CAPTURE = false, bool STRICT = false>
 LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, double steering,
                                        double throttle, double braking, double dt, X18* x,
                                        TireOut18* to = nullptr) {
@@ -111,10 +115,11 @@ LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, doubl
     // computeTireLoads
     const double lat = s.vdot + s.wz * s.u;
     const double Z2f = k.Z2fc * lat;
-    const double Z3f = (P.krof * s.phi + P.brof * s.wx) * u.inv_cf;
-    const double Z4 = (k.Z4n * (s.udot - s.wz * s.v)) * u.inv_two_ab;
+    const double Z3f = STRICT ? qdiv(P.krof * s.phi + P.brof * s.wx, P.cf) : (P.krof * s.phi + P.brof * s.wx) * u.inv_cf;
+    const double Z4 = STRICT ? qdiv(k.Z4n * (s.udot - s.wz * s.v), u.two_ab)
+                             : (k.Z4n * (s.udot - s.wz * s.v)) * u.inv_two_ab;
     const double Z2r = k.Z2rc * lat;
-    const double Z3r = (P.kror * s.phi + P.bror * s.wx) * u.inv_cr;
+    const double Z3r = STRICT ? qdiv(P.kror * s.phi + P.bror * s.wx, P.cr) : (P.kror * s.phi + P.bror * s.wx) * u.inv_cr;
     double fz[4];
     fz[0] = k.Z1f - Z2f - Z3f - Z4;
     fz[1] = k.Z1f + Z2f + Z3f - Z4;
@@ -139,12 +144,14 @@ LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, doubl
     double fx[4], fy[4], My[4], xt[4], rStat[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
+        const double dl = i < 2 ? delta : u.delta_rear;
         const double d = i < 2 ? qdiv(sd, cd) : u.tan_delta_rear;
         if (!NR)
-            tire_tmeasy(t, u, k, d, fz[i], vsx[i], vsy[i], s.xe[i], s.ye[i], s.xedot[i], s.yedot[i], s.omega[i],
-                        fx[i], fy[i], My[i], xt[i], rStat[i]);
+            tire_tmeasy<STRICT>(t, u, k, dl, d, fz[i], vsx[i], vsy[i], s.xe[i], s.ye[i], s.xedot[i], s.yedot[i],
+                                s.omega[i], fx[i], fy[i], My[i], xt[i], rStat[i]);
         else
-            tire_tmeasy_nr(t, u, k, d, fz[i], vsx[i], vsy[i], s.omega[i], fx[i], fy[i], My[i], xt[i], rStat[i]);
+            tire_tmeasy_nr<STRICT>(t, u, k, dl, d, fz[i], vsx[i], vsy[i], s.omega[i], fx[i], fy[i], My[i], xt[i],
+                                   rStat[i]);
     }
     if (CAPTURE) {
 #pragma unroll
@@ -176,12 +183,15 @@ LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, doubl
                                                  k.two_hrc_m * s.wz * s.wx);
     const double A1 = u.A1, A2 = k.A2, A3 = k.A3, jz = k.jz, jxz = P.jxz, mt = -k.neg_mt;
     const double A3E1 = A3 * E1;
-    const double vdot = (E1 * u.jxz2 - k.A1A2 * E2 + A1 * E3 * jxz + A3 * E2 * jxz - A2 * E1 * jz - A3 * E3 * jz) *
-                        k.rdenom;
-    const double wxdot =
-        (u.A1sq * E3 - k.A1A3 * E2 + A1 * E1 * jxz - A3E1 * jz + E2 * jxz * mt - E3 * jz * mt) * k.rdenom;
-    const double wzdot =
-        (k.A3sq * E2 - k.A1A2 * E1 - k.A1A3 * E3 + A3E1 * jxz - A2 * E2 * mt + E3 * jxz * mt) * k.rdenom;
+    const double nv = E1 * u.jxz2 - k.A1A2 * E2 + A1 * E3 * jxz + A3 * E2 * jxz - A2 * E1 * jz - A3 * E3 * jz;
+    const double nwx = u.A1sq * E3 - k.A1A3 * E2 + A1 * E1 * jxz - A3E1 * jz + E2 * jxz * mt - E3 * jz * mt;
+    const double nwz = k.A3sq * E2 - k.A1A2 * E1 - k.A1A3 * E3 + A3E1 * jxz - A2 * E2 * mt + E3 * jxz * mt;
+    // the strict step divides by the determinant, formed as k18_init forms it (A1sq, A3sq and jxz2 are
+    // the products it squares)
+    const double denom = STRICT ? A2 * u.A1sq - 2. * A1 * A3 * jxz + jz * k.A3sq + mt * u.jxz2 - A2 * jz * mt : 0.0;
+    const double vdot = STRICT ? qdiv(nv, denom) : nv * k.rdenom;
+    const double wxdot = STRICT ? qdiv(nwx, denom) : nwx * k.rdenom;
+    const double wzdot = STRICT ? qdiv(nwz, denom) : nwz * k.rdenom;
     // Half-implicit update (dof18_halfImplicit.cpp:262-305)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {

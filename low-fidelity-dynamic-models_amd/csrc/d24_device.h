@@ -59,7 +59,13 @@ LFDM_HD void k24_init(const PSet& P, double m, double jz, double jx, double dfy0
 }
 
 // One half-implicit step (rhsFun order, dof24_halfImplicit.cpp:594-706; update :188-297).
-template <bool NR, bool FULL>
+// STRICT: the reference's own quotients and transcendentals throughout (see dev_tire.h): the static
+// radius divided by cos(theta) cos(phi), the jacking forces by the track widths, the Euler-angle
+// rates by cos(theta) with tan(theta) from the libm.  Where a reference function uses only the sine
+// or only the cosine of an angle (computeTireRHS: cos theta and cos phi; computeForcesThroughSus:
+// sin phi; the Euler-angle rates of the step: cos theta), the strict step calls that function alone;
+// everywhere else the reference's sin and cos of one angle compile to one sincos, as here.
+template <bool NR, bool FULL, bool STRICT = false>
 LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, double steering,
                                        double throttle, double braking, double dt, X24* x) {
     const TireP& t = P.tire;
@@ -118,20 +124,21 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     const double rcc = qdiv(1., cc);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
+        const double dl = i < 2 ? delta : u.delta_rear;
         const double d = i < 2 ? qdiv(sd, cd) : u.tan_delta_rear;
         fz[i] = s.xt[i] * t.kt;
-        s.rStat[i] = (t.r0 - s.xt[i]) * rcc;
+        s.rStat[i] = STRICT ? qdiv(t.r0 - s.xt[i], cos(s.theta) * cos(s.phi)) : (t.r0 - s.xt[i]) * rcc;
         if (fz[i] <= 0) {
             if (!NR) { s.xe[i] = 0; s.ye[i] = 0; }
             s.omega[i] = 0; s.xt[i] = 0; s.rStat[i] = t.r0; fz[i] = 0; fx[i] = 0; fy[i] = 0; My[i] = 0;
             continue;
         }
         if (!NR)
-            tmeasy_core(t, u, k.qy1, k.qy2, d, fz[i], s.rStat[i], vsx[i], vsy[i], s.xe[i], s.ye[i], s.omega[i],
-                        s.xedot[i], s.yedot[i], fx[i], fy[i], My[i]);
+            tmeasy_core<STRICT>(t, u, k.qy1, k.qy2, dl, d, fz[i], s.rStat[i], vsx[i], vsy[i], s.xe[i], s.ye[i],
+                                s.omega[i], s.xedot[i], s.yedot[i], fx[i], fy[i], My[i]);
         else
-            tmeasy_nr_core(t, u, k.qy1, k.qy2, d, fz[i], s.rStat[i], vsx[i], vsy[i], s.omega[i], fx[i], fy[i],
-                           My[i]);
+            tmeasy_nr_core<STRICT>(t, u, k.qy1, k.qy2, dl, d, fz[i], s.rStat[i], vsx[i], vsy[i], s.omega[i], fx[i],
+                                   fy[i], My[i]);
     }
     // computeTireCompressionVelocity (dof24.cpp:1122-1151)
     double dxt[4];
@@ -170,24 +177,25 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     // computeForcesThroughSus (dof24.cpp:599-680); the rear-right lateral force has no cos(theta)
     // factor in the reference (:639) and keeps that form here
     double fxs[4], fys[4], fzs[4];
+    const double sph_f = STRICT ? sin(s.phi) : sph;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int ax = i < 2 ? 0 : 1;
         const double mu = ax == 0 ? P.muf : P.mur;
         fxs[i] = fxg[i] + k.mu_g[ax] * sth - mu * duu[i] + mu * s.wz * vu[i] - mu * s.wy * s.wu[i];
         if (i < 3)
-            fys[i] = fyg[i] - k.mu_g[ax] * sph * cth - mu * dvu[i] + mu * s.wx * s.wu[i] - mu * s.wz * uu[i];
+            fys[i] = fyg[i] - k.mu_g[ax] * sph_f * cth - mu * dvu[i] + mu * s.wx * s.wu[i] - mu * s.wz * uu[i];
         else
-            fys[i] = fyg[i] - k.mu_g[ax] * sph - mu * dvu[i] + mu * s.wx * s.wu[i] - mu * s.wz * uu[i];
+            fys[i] = fyg[i] - k.mu_g[ax] * sph_f - mu * dvu[i] + mu * s.wx * s.wu[i] - mu * s.wz * uu[i];
         fzs[i] = s.xs[i] * P.ks + s.dxs[i] * P.bs;
     }
-    s.fzd[1] = (fyg[1] * s.rStat[1] + fys[1] * ls[1] + fyg[0] * s.rStat[0] + fys[0] * ls[0] -
-                (fys[1] + fys[0]) * P.hrcf) *
-               u.inv_cf;
+    const double jf = fyg[1] * s.rStat[1] + fys[1] * ls[1] + fyg[0] * s.rStat[0] + fys[0] * ls[0] -
+                      (fys[1] + fys[0]) * P.hrcf;
+    s.fzd[1] = STRICT ? qdiv(jf, cf) : jf * u.inv_cf;
     s.fzd[0] = -s.fzd[1];
-    s.fzd[3] = (fyg[3] * s.rStat[3] + fys[3] * ls[3] + fyg[2] * s.rStat[2] + fys[2] * ls[2] -
-                (fys[3] + fys[2]) * P.hrcr) *
-               u.inv_cr;
+    const double jr = fyg[3] * s.rStat[3] + fys[3] * ls[3] + fyg[2] * s.rStat[2] + fys[2] * ls[2] -
+                      (fys[3] + fys[2]) * P.hrcr;
+    s.fzd[3] = STRICT ? qdiv(jr, cr) : jr * u.inv_cr;
     s.fzd[2] = -s.fzd[3];
     double mx[4], my[4];
     mx[0] = fys[0] * P.hrcf;
@@ -233,9 +241,11 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     // Euler-angle rates at the step's (old) theta and phi: tan(theta) and 1/cos(theta) from the
     // step's sine and cosine (same values, other rounding than tan() and two divisions)
     const double rcth = qdiv(1., cth);
-    const double tth = sth * rcth;
+    const double tth = STRICT ? tan(s.theta) : sth * rcth;
     const double dtheta = s.wy * cph - s.wz * sph;
-    const double dpsi = (s.wy * sph * rcth) + (s.wz * cph * rcth);
+    const double cth_e = STRICT ? cos(s.theta) : cth;
+    const double dpsi =
+        STRICT ? qdiv(s.wy * sph, cth_e) + qdiv(s.wz * cph, cth_e) : (s.wy * sph * rcth) + (s.wz * cph * rcth);
     const double dphi = s.wx + s.wy * sph * tth + s.wz * cph * tth;
     s.theta = s.theta + dtheta * dt;
     s.psi = s.psi + dpsi * dt;

@@ -9,7 +9,9 @@
 namespace lfdm {
 namespace dev {
 
-// tmxy_combined (dof18.cpp:247-288); returns fos, and f through `f`.
+// tmxy_combined (dof18.cpp:247-288); returns fos, and f through `f`.  STRICT divides by the
+// adhesion-branch denominator twice, as the reference writes it; otherwise once, as a reciprocal.
+template <bool STRICT = false>
 LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, double fs,
                                            double& f) {
     double df0loc = 0.0;
@@ -25,9 +27,15 @@ LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, d
         } else if (s < sm) {
             const double p = qdiv(df0loc * sm, fm) - 2.0;
             const double sn = qdiv(s, sm);
-            const double rdn = qdiv(1.0, 1.0 + (sn + p) * sn);
-            f = df0loc * sm * sn * rdn;
-            fos = df0loc * rdn;
+            const double dn = 1.0 + (sn + p) * sn;
+            if (STRICT) {
+                f = qdiv(df0loc * sm * sn, dn);
+                fos = qdiv(df0loc, dn);
+            } else {
+                const double rdn = qdiv(1.0, dn);
+                f = df0loc * sm * sn * rdn;
+                fos = df0loc * rdn;
+            }
         } else {
             const double q = qdiv(fm, sm);
             const double a = qdiv(q * q, df0loc * sm);
@@ -57,10 +65,12 @@ struct Curve {
     double dfx0, dfy0, fxm, fym, fxs, fys, sxm, sym, sxs, sys;
 };
 
-// qy1, qy2: InterpQ coefficients of the (per-vehicle) initial lateral slope dfy0.
+// qy1, qy2: InterpQ coefficients of the (per-vehicle) initial lateral slope dfy0.  STRICT forms
+// fz / pn as the reference does; otherwise fz times the set's 1 / pn.
+template <bool STRICT = false>
 LFDM_HD void curve_at(const TireP& t, const Derived& u, double qy1, double qy2, double fz,
                                          Curve& c) {
-    const double fzn = fz * u.inv_pn;
+    const double fzn = STRICT ? qdiv(fz, t.pn) : fz * u.inv_pn;
     const double fzn1 = fzn - 1.;
     // InterpQ(fz, w1, w2, pn) = (fz/pn) * (2 w1 - 0.5 w2 - (w1 - 0.5 w2) (fz/pn)), utils.h:98-100
     c.dfx0 = fzn * (u.qx1 - u.qx2 * fzn);
@@ -87,7 +97,17 @@ LFDM_HD void curve_at(const TireP& t, const Derived& u, double qy1, double qy2, 
 //    away from the overflow and underflow that hypot's scaling guards against;
 //  - divisions by a shared denominator (vta, hsxn, hsyn, sc, sxm + sym, rx + ry, pn, kt) as
 //    multiplications by its reciprocal.
+// The STRICT instantiations evaluate all three as the reference writes them: atan2 and tan of the
+// steering angle, the device libm's hypot, and every quotient as its own (correctly rounded)
+// division, so that they differ from the reference only where the device libm and glibc round
+// differently.
 LFDM_HD double hyp(double a, double b) { return qsqrt(a * a + b * b); }
+
+template <bool STRICT>
+LFDM_HD double hyp2(double a, double b) {
+    if (STRICT) return hypot(a, b);
+    return hyp(a, b);
+}
 
 // Intermediate values of the TMeasy force core (dof18.cpp:412-539; the same code is
 // dof11.cpp:335-472 and dof24.cpp:851-987) for a tire on the ground (fz > 0) with static radius
@@ -104,12 +124,14 @@ struct TireEval {
     double fxdyn, fydyn, fxr, fyr, fxstr, fystr, avx, avy, wxw, wyw, fx, fy, My;
 };
 
-// vsx, vsy are the contact-patch velocities of the transform, tan_delta the tangent of the tire's
-// steering angle.
-LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy2, double tan_delta, double fz,
-                         double rStat, double vsx, double vsy, double xe, double ye, double omega, TireEval& w) {
+// vsx, vsy are the contact-patch velocities of the transform, delta the tire's steering angle and
+// tan_delta its tangent (the fast form reads only the tangent, the strict one only the angle).
+template <bool STRICT = false>
+LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy2, double delta, double tan_delta,
+                         double fz, double rStat, double vsx, double vsy, double xe, double ye, double omega,
+                         TireEval& w) {
     if (fz <= t.fzRdynco) {
-        w.rdynco = t.rdyncoPn + u.lrd * (fz * u.inv_pn - 1.);
+        w.rdynco = t.rdyncoPn + u.lrd * ((STRICT ? qdiv(fz, t.pn) : fz * u.inv_pn) - 1.);
         w.drd = u.lrd * u.inv_pn;
     } else {
         w.rdynco = t.rdyncoCrit;
@@ -121,52 +143,82 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
     w.som = sgn(omega);
     w.rw = w.r_eff * w.aom;
     w.vta = w.rw + 0.01;
-    const double rvta = qdiv(1., w.vta);
-    w.sx = -w.vsx2 * rvta;
-    const double r = vsy * rvta;  // tan(atan2(vsy, vta))
-    w.sy = -qdiv(r - tan_delta, 1. + r * tan_delta);
+    if (STRICT) {
+        w.sx = qdiv(-w.vsx2, w.vta);
+        w.sy = -tan(atan2(vsy, w.vta) - delta);
+    } else {
+        const double rvta = qdiv(1., w.vta);
+        w.sx = -w.vsx2 * rvta;
+        const double r = vsy * rvta;  // tan(atan2(vsy, vta))
+        w.sy = -qdiv(r - tan_delta, 1. + r * tan_delta);
+    }
     w.clip = fz > t.pnmax;
     w.fzc = w.clip ? t.pnmax : fz;
     const Curve& c = w.c;
-    curve_at(t, u, qy1, qy2, w.fzc, w.c);
+    curve_at<STRICT>(t, u, qy1, qy2, w.fzc, w.c);
     w.fzn = w.fzc * u.inv_pn;
     w.rx = qdiv(c.fxm, c.dfx0);
     w.ry = qdiv(c.fym, c.dfy0);
     w.ssum = c.sxm + c.sym;
     w.rsum = w.rx + w.ry;
-    w.rssum = qdiv(1., w.ssum);
-    w.rrsum = qdiv(1., w.rsum);
-    w.hsxn = c.sxm * w.rssum + w.rx * w.rrsum;
-    w.hsyn = c.sym * w.rssum + w.ry * w.rrsum;
-    w.rhsxn = qdiv(1., w.hsxn);
-    w.rhsyn = qdiv(1., w.hsyn);
-    w.sxn = w.sx * w.rhsxn;
-    w.syn = w.sy * w.rhsyn;
-    w.sc = hyp(w.sxn, w.syn);
-    if (w.sc > 0) {
-        const double rsc = qdiv(1., w.sc);
-        w.ca = w.sxn * rsc;
-        w.sa = w.syn * rsc;
+    if (STRICT) {
+        w.hsxn = qdiv(c.sxm, w.ssum) + qdiv(w.rx, w.rsum);
+        w.hsyn = qdiv(c.sym, w.ssum) + qdiv(w.ry, w.rsum);
+        w.sxn = qdiv(w.sx, w.hsxn);
+        w.syn = qdiv(w.sy, w.hsyn);
+        w.sc = hypot(w.sxn, w.syn);
+        if (w.sc > 0) {
+            w.ca = qdiv(w.sxn, w.sc);
+            w.sa = qdiv(w.syn, w.sc);
+        } else {
+            w.ca = kHalfSqrt2;
+            w.sa = kHalfSqrt2;
+        }
     } else {
-        w.ca = kHalfSqrt2;
-        w.sa = kHalfSqrt2;
+        w.rssum = qdiv(1., w.ssum);
+        w.rrsum = qdiv(1., w.rsum);
+        w.hsxn = c.sxm * w.rssum + w.rx * w.rrsum;
+        w.hsyn = c.sym * w.rssum + w.ry * w.rrsum;
+        w.rhsxn = qdiv(1., w.hsxn);
+        w.rhsyn = qdiv(1., w.hsyn);
+        w.sxn = w.sx * w.rhsxn;
+        w.syn = w.sy * w.rhsyn;
+        w.sc = hyp(w.sxn, w.syn);
+        if (w.sc > 0) {
+            const double rsc = qdiv(1., w.sc);
+            w.ca = w.sxn * rsc;
+            w.sa = w.syn * rsc;
+        } else {
+            w.ca = kHalfSqrt2;
+            w.sa = kHalfSqrt2;
+        }
     }
     w.A = c.dfx0 * w.ca * w.hsxn;
     w.B = c.dfy0 * w.sa * w.hsyn;
     w.C = c.fxm * w.ca;
     w.D = c.fym * w.sa;
-    w.E = c.sxm * w.ca * w.rhsxn;
-    w.F = c.sym * w.sa * w.rhsyn;
+    if (STRICT) {
+        w.E = qdiv(c.sxm * w.ca, w.hsxn);
+        w.F = qdiv(c.sym * w.sa, w.hsyn);
+    } else {
+        w.E = c.sxm * w.ca * w.rhsxn;
+        w.F = c.sym * w.sa * w.rhsyn;
+    }
     w.G = c.fxs * w.ca;
     w.H = c.fys * w.sa;
-    w.I = c.sxs * w.ca * w.rhsxn;
-    w.J = c.sys * w.sa * w.rhsyn;
-    w.df0 = hyp(w.A, w.B);
-    w.fm = hyp(w.C, w.D);
-    w.sm = hyp(w.E, w.F);
-    w.fs = hyp(w.G, w.H);
-    w.ss = hyp(w.I, w.J);
-    w.fos = tmxy_fos(w.sc, w.df0, w.sm, w.fm, w.ss, w.fs, w.f);
+    if (STRICT) {
+        w.I = qdiv(c.sxs * w.ca, w.hsxn);
+        w.J = qdiv(c.sys * w.sa, w.hsyn);
+    } else {
+        w.I = c.sxs * w.ca * w.rhsxn;
+        w.J = c.sys * w.sa * w.rhsyn;
+    }
+    w.df0 = hyp2<STRICT>(w.A, w.B);
+    w.fm = hyp2<STRICT>(w.C, w.D);
+    w.sm = hyp2<STRICT>(w.E, w.F);
+    w.fs = hyp2<STRICT>(w.G, w.H);
+    w.ss = hyp2<STRICT>(w.I, w.J);
+    w.fos = tmxy_fos<STRICT>(w.sc, w.df0, w.sm, w.fm, w.ss, w.fs, w.f);
     // -sineStep(vta, 0, 0, 0, 1) * rr * fz * rStat * sgn(omega) with the limited fz; the step is 0
     // for vta <= 0, else 1
     w.ssv = (w.vta <= 0.0) ? 0.0 : 1.0;
@@ -180,7 +232,8 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
     w.deny = w.vtys * t.dy + w.fos;
     w.xedot = qdiv(w.numx, w.denx);
     w.yedot = qdiv(w.numy, w.deny);
-    w.fxdyn = t.dx * w.xedot + t.cx * xe;
+    // the reference forms dx * num / den as (dx * num) / den here, but dy * (num / den) below
+    w.fxdyn = (STRICT ? qdiv(t.dx * w.numx, w.denx) : t.dx * w.xedot) + t.cx * xe;
     w.fydyn = t.dy * w.yedot + (t.cy * ye);
     w.fxr = xe * t.cx + w.xedot * t.dx;
     w.fyr = ye * t.cy + w.yedot * t.dy;
@@ -195,11 +248,12 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
 }
 
 // TMeasy force core: deflection rates, forces and rolling moment of a tire on the ground.
-LFDM_HD void tmeasy_core(const TireP& t, const Derived& u, double qy1, double qy2, double tan_delta, double fz,
-                         double rStat, double vsx, double vsy, double xe, double ye, double omega, double& xedot,
-                         double& yedot, double& fx, double& fy, double& My) {
+template <bool STRICT = false>
+LFDM_HD void tmeasy_core(const TireP& t, const Derived& u, double qy1, double qy2, double delta, double tan_delta,
+                         double fz, double rStat, double vsx, double vsy, double xe, double ye, double omega,
+                         double& xedot, double& yedot, double& fx, double& fy, double& My) {
     TireEval w;
-    tmeasy_eval(t, u, qy1, qy2, tan_delta, fz, rStat, vsx, vsy, xe, ye, omega, w);
+    tmeasy_eval<STRICT>(t, u, qy1, qy2, delta, tan_delta, fz, rStat, vsx, vsy, xe, ye, omega, w);
     xedot = w.xedot;
     yedot = w.yedot;
     fx = w.fx;
@@ -209,13 +263,14 @@ LFDM_HD void tmeasy_core(const TireP& t, const Derived& u, double qy1, double qy
 
 // TMeasyNr force core (dof18.cpp:571-673 with the Coulomb blend of :290-306; the same code is
 // dof11.cpp:503-606 and dof24.cpp:1014-1121) for a tire on the ground with static radius rStat.
-LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double qy2,
+template <bool STRICT = false>
+LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double qy2, double delta,
                                                double tan_delta, double fz, double rStat, double vsx, double vsy,
                                                double omega, double& fx, double& fy, double& My) {
     const double vsx_in = vsx;
     double rdynco;
     if (fz <= t.fzRdynco)
-        rdynco = t.rdyncoPn + u.lrd * (fz * u.inv_pn - 1.);
+        rdynco = t.rdyncoPn + u.lrd * ((STRICT ? qdiv(fz, t.pn) : fz * u.inv_pn) - 1.);
     else
         rdynco = t.rdyncoCrit;
     const double r_eff = rdynco * t.r0 + (1. - rdynco) * rStat;
@@ -223,39 +278,48 @@ LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double
     const double vta = r_eff * fabs(omega) + 0.01;
     double Fx0 = tanh(qdiv(-2.0 * vsx, t.vcoulomb)) * fz * t.mu;
     double Fy0 = tanh(qdiv(-2.0 * vsy, t.vcoulomb)) * fz * t.mu;
-    const double F0 = hyp(Fx0, Fy0);
+    const double F0 = hyp2<STRICT>(Fx0, Fy0);
     if (F0 > fz * t.mu) {
         const double fsc = qdiv(fz * t.mu, F0);
         Fx0 = Fx0 * fsc;
         Fy0 = Fy0 * fsc;
     }
-    const double rvta = qdiv(1., vta);
-    const double sx = -vsx * rvta;
-    const double r = vsy * rvta;  // tan(atan2(vsy, vta))
-    const double sy = -qdiv(r - tan_delta, 1. + r * tan_delta);
+    double sx, sy;
+    if (STRICT) {
+        sx = qdiv(-vsx, vta);
+        sy = -tan(atan2(vsy, vta) - delta);
+    } else {
+        const double rvta = qdiv(1., vta);
+        sx = -vsx * rvta;
+        const double r = vsy * rvta;  // tan(atan2(vsy, vta))
+        sy = -qdiv(r - tan_delta, 1. + r * tan_delta);
+    }
     if (fz > t.pnmax) fz = t.pnmax;
     Curve c;
-    curve_at(t, u, qy1, qy2, fz, c);
+    curve_at<STRICT>(t, u, qy1, qy2, fz, c);
     const double frblend = sine_step(fabs(vsx_in), t.frblend_begin, 0., t.frblend_end, 1.);
-    const double sc = hyp(sx, sy);
+    const double sc = hyp2<STRICT>(sx, sy);
     double ca, sa;
-    const double rsc = sc > 0 ? qdiv(1., sc) : 0.;
+    const double rsc = (!STRICT && sc > 0) ? qdiv(1., sc) : 0.;
     if (sc > 0) {
-        ca = sx * rsc;
-        sa = sy * rsc;
+        ca = STRICT ? qdiv(sx, sc) : sx * rsc;
+        sa = STRICT ? qdiv(sy, sc) : sy * rsc;
     } else {
         ca = kHalfSqrt2;
         sa = kHalfSqrt2;
     }
-    const double df0 = hyp(c.dfx0 * ca, c.dfy0 * sa);
-    const double fm = t.mu * hyp(c.fxm * ca, c.fym * sa);
-    const double sm = t.mu * hyp(c.sxm * ca, c.sym * sa);
-    const double fs = t.mu * hyp(c.fxs * ca, c.fys * sa);
-    const double ss = t.mu * hyp(c.sxs * ca, c.sys * sa);
+    const double df0 = hyp2<STRICT>(c.dfx0 * ca, c.dfy0 * sa);
+    const double fm = t.mu * hyp2<STRICT>(c.fxm * ca, c.fym * sa);
+    const double sm = t.mu * hyp2<STRICT>(c.sxm * ca, c.sym * sa);
+    const double fs = t.mu * hyp2<STRICT>(c.fxs * ca, c.fys * sa);
+    const double ss = t.mu * hyp2<STRICT>(c.sxs * ca, c.sys * sa);
     double f;
-    tmxy_fos(sc, df0, sm, fm, ss, fs, f);
+    tmxy_fos<STRICT>(sc, df0, sm, fm, ss, fs, f);
     double Fx, Fy;
-    if (sc > 0.) {
+    if (sc > 0. && STRICT) {
+        Fx = qdiv(f * sx, sc);
+        Fy = qdiv(f * sy, sc);
+    } else if (sc > 0.) {
         Fx = f * sx * rsc;
         Fy = f * sy * rsc;
     } else {

@@ -77,6 +77,7 @@ __device__ __forceinline__ Ctl controls_at(const Launch& L, const PSet& P, CtrlS
 }
 
 // ---- model traits: record layout, state load/store, output channels ----
+// (load, kinit, step and store also compile for the host, where the host checks run the step code)
 
 // 18-DOF record: per tire LF RF LR RR (TMeasy: xe ye xedot yedot) omega dOmega xt rStat fx fy fz vsx
 // vsy My engTor; vehicle x y u v psi wz phi wx udot vdot wxdot wzdot crankOmega dOmega_crank gear.
@@ -89,10 +90,10 @@ struct M18 {
     template <bool NR>
     struct R {
         static constexpr int kTire = NR ? 11 : 15, kVeh = 4 * kTire, om = NR ? 0 : 4;
-        static __device__ __forceinline__ int f(int i, int o) { return i * kTire + om + o; }  // o: omega = 0 ...
+        static LFDM_HD int f(int i, int o) { return i * kTire + om + o; }  // o: omega = 0 ...
     };
     template <bool NR>
-    static __device__ __forceinline__ void load(const double* st, size_t S, int slot, S18& s) {
+    static LFDM_HD void load(const double* st, size_t S, int slot, S18& s) {
         using Rr = R<NR>;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -111,13 +112,13 @@ struct M18 {
         s.phi = v[6 * S]; s.wx = v[7 * S]; s.udot = v[8 * S]; s.vdot = v[9 * S]; s.crank = v[12 * S];
         s.gear = (int)v[14 * S];
     }
-    static __device__ __forceinline__ void kinit(const PSet& P, const double* o, size_t S, K18& k) {
+    static LFDM_HD void kinit(const PSet& P, const double* o, size_t S, K18& k) {
         k18_init(P, o[0], o[S], o[2 * S], o[3 * S], o[4 * S], o[5 * S], o[6 * S], k);
     }
-    template <bool NR, bool FULL>
-    static __device__ __forceinline__ void step(const PSet& P, const Derived& u, const K18& k, S18& s, const Ctl& c,
+    template <bool NR, bool FULL, bool STRICT>
+    static LFDM_HD void step(const PSet& P, const Derived& u, const K18& k, S18& s, const Ctl& c,
                                                 double dt, X18* x) {
-        step18<NR, FULL>(P, u, k, s, c.s, c.th, c.b, dt, x);
+        step18<NR, FULL, false, STRICT>(P, u, k, s, c.s, c.th, c.b, dt, x);
     }
     static __device__ __forceinline__ void emit(double* o, size_t nu, double t, const S18& s) {
         o[0 * nu] = t; o[1 * nu] = s.x; o[2 * nu] = s.y; o[3 * nu] = s.u; o[4 * nu] = s.v;
@@ -125,7 +126,7 @@ struct M18 {
         o[9 * nu] = s.omega[0]; o[10 * nu] = s.omega[1]; o[11 * nu] = s.omega[2]; o[12 * nu] = s.omega[3];
     }
     template <bool NR>
-    static __device__ __forceinline__ void store(double* w, size_t S, int slot, const PSet& P, const S18& s,
+    static LFDM_HD void store(double* w, size_t S, int slot, const PSet& P, const S18& s,
                                                  const X18& x) {
         using Rr = R<NR>;
 #pragma unroll
@@ -161,10 +162,10 @@ struct M11 {
     template <bool NR>
     struct R {
         static constexpr int kTire = NR ? 11 : 15, kVeh = 2 * kTire, om = NR ? 0 : 4;
-        static __device__ __forceinline__ int f(int i, int o) { return i * kTire + om + o; }
+        static LFDM_HD int f(int i, int o) { return i * kTire + om + o; }
     };
     template <bool NR>
-    static __device__ __forceinline__ void load(const double* st, size_t S, int slot, S11& s) {
+    static LFDM_HD void load(const double* st, size_t S, int slot, S11& s) {
         using Rr = R<NR>;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
@@ -183,20 +184,20 @@ struct M11 {
         s.udot = v[6 * S]; s.crank = v[9 * S];
         s.gear = (int)v[11 * S];
     }
-    static __device__ __forceinline__ void kinit(const PSet& P, const double* o, size_t S, K11& k) {
+    static LFDM_HD void kinit(const PSet& P, const double* o, size_t S, K11& k) {
         k11_init(P, o[0], o[S], o[3 * S], o[4 * S], o[5 * S], o[6 * S], k);
     }
-    template <bool NR, bool FULL>
-    static __device__ __forceinline__ void step(const PSet& P, const Derived& u, const K11& k, S11& s, const Ctl& c,
+    template <bool NR, bool FULL, bool STRICT>
+    static LFDM_HD void step(const PSet& P, const Derived& u, const K11& k, S11& s, const Ctl& c,
                                                 double dt, X11* x) {
-        step11<NR, FULL>(P, u, k, s, c.s, c.th, c.b, dt, x);
+        step11<NR, FULL, STRICT>(P, u, k, s, c.s, c.th, c.b, dt, x);
     }
     static __device__ __forceinline__ void emit(double* o, size_t nu, double t, const S11& s) {
         o[0 * nu] = t; o[1 * nu] = s.x; o[2 * nu] = s.y; o[3 * nu] = s.u; o[4 * nu] = s.v;
         o[5 * nu] = s.psi; o[6 * nu] = s.wz; o[7 * nu] = s.omega[0]; o[8 * nu] = s.omega[1];
     }
     template <bool NR>
-    static __device__ __forceinline__ void store(double* w, size_t S, int slot, const PSet& P, const S11& s,
+    static LFDM_HD void store(double* w, size_t S, int slot, const PSet& P, const S11& s,
                                                  const X11& x) {
         using Rr = R<NR>;
 #pragma unroll
@@ -233,13 +234,13 @@ struct M24 {
     template <bool NR>
     struct R {
         static constexpr int kTire = NR ? 19 : 23, kSus = 4 * kTire, kVeh = kSus + 4 * 24, om = NR ? 0 : 4;
-        static __device__ __forceinline__ int f(int i, int o) { return i * kTire + om + o; }
-        static __device__ __forceinline__ int q(int i, int o) { return kSus + i * 24 + o; }
+        static LFDM_HD int f(int i, int o) { return i * kTire + om + o; }
+        static LFDM_HD int q(int i, int o) { return kSus + i * 24 + o; }
     };
     // tire fields after omega: dOmega 1, xt 2, dxt 3, rStat 4, fx 5, fy 6, fz 7, fxg 8, fyg 9, fzg 10,
     // vgx 11, vgy 12, vgz 13, vsx 14, vsy 15, vsz 16, My 17, engTor 18
     template <bool NR>
-    static __device__ __forceinline__ void load(const double* st, size_t S, int slot, S24& s) {
+    static LFDM_HD void load(const double* st, size_t S, int slot, S24& s) {
         using Rr = R<NR>;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -269,13 +270,13 @@ struct M24 {
         s.wzdot = v[17 * S]; s.crank = v[21 * S];
         s.gear = (int)v[23 * S];
     }
-    static __device__ __forceinline__ void kinit(const PSet& P, const double* o, size_t S, K24& k) {
+    static LFDM_HD void kinit(const PSet& P, const double* o, size_t S, K24& k) {
         k24_init(P, o[0], o[S], o[2 * S], o[3 * S], o[4 * S], o[5 * S], o[6 * S], k);
     }
-    template <bool NR, bool FULL>
-    static __device__ __forceinline__ void step(const PSet& P, const Derived& u, const K24& k, S24& s, const Ctl& c,
+    template <bool NR, bool FULL, bool STRICT>
+    static LFDM_HD void step(const PSet& P, const Derived& u, const K24& k, S24& s, const Ctl& c,
                                                 double dt, X24* x) {
-        step24<NR, FULL>(P, u, k, s, c.s, c.th, c.b, dt, x);
+        step24<NR, FULL, STRICT>(P, u, k, s, c.s, c.th, c.b, dt, x);
     }
     static __device__ __forceinline__ void emit(double* o, size_t nu, double t, const S24& s) {
         o[0 * nu] = t; o[1 * nu] = s.x; o[2 * nu] = s.y; o[3 * nu] = s.u; o[4 * nu] = s.v;
@@ -284,7 +285,7 @@ struct M24 {
         o[14 * nu] = s.omega[3];
     }
     template <bool NR>
-    static __device__ __forceinline__ void store(double* w, size_t S, int slot, const PSet& P, const S24& s,
+    static LFDM_HD void store(double* w, size_t S, int slot, const PSet& P, const S24& s,
                                                  const X24& x) {
         using Rr = R<NR>;
 #pragma unroll
@@ -333,8 +334,10 @@ struct M24 {
 // slower); the 24-DOF step would spill over a kilobyte per lane at two waves and runs 2.6x faster
 // at one.  A fleet that fills at most one wavefront per SIMD gains nothing from the other waves'
 // register budget, so every kernel also exists at one wave per SIMD, and the 11-DOF one at two;
-// Launch::waves picks (0: the model's default).
-template <class M, bool NR, int MODE, int W>
+// Launch::waves picks (0: the model's default).  STRICT instantiations evaluate the step with the
+// reference's own quotients and transcendentals (Launch::strict; built at the model's default
+// occupancy only).
+template <class M, bool NR, int MODE, int W, bool STRICT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) void integrate(Launch L) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int wave = uniform(slot >> 6);
@@ -371,7 +374,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
     for (int kk = 0; kk < wsteps - 1; ++kk) {
         if (kk < my - 1) {
             const Ctl c = controls_at<MODE>(L, P, cs, uv, kk, t);
-            M::template step<NR, false>(P, u, k, s, c, dt, nullptr);
+            M::template step<NR, false, STRICT>(P, u, k, s, c, dt, nullptr);
             t = t + dt;
         }
         if (kk + 1 == next_out) {
@@ -387,7 +390,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
         typename M::X x;
         CtrlState<MODE> cl;
         const Ctl c = controls_at<MODE>(L, P, my == wsteps ? cs : cl, uv, my - 1, t);
-        M::template step<NR, true>(P, u, k, s, c, dt, &x);
+        M::template step<NR, true, STRICT>(P, u, k, s, c, dt, &x);
         t = t + dt;
         M::template store<NR>(L.state, S, slot, P, s, x);
     }
@@ -404,26 +407,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
     }
 }
 
-template <class M, bool NR, int W>
+template <class M, bool NR, int W, bool STRICT>
 hipError_t launch_mode(const Launch& L, int block, hipStream_t stream) {
     const int grid = (L.S + block - 1) / block;
     switch (L.mode) {
-        case kCtrlTable: hipLaunchKernelGGL((integrate<M, NR, kCtrlTable, W>), grid, block, 0, stream, L); break;
-        case kCtrlConst: hipLaunchKernelGGL((integrate<M, NR, kCtrlConst, W>), grid, block, 0, stream, L); break;
-        case kCtrlKnots: hipLaunchKernelGGL((integrate<M, NR, kCtrlKnots, W>), grid, block, 0, stream, L); break;
-        case kCtrlSteps: hipLaunchKernelGGL((integrate<M, NR, kCtrlSteps, W>), grid, block, 0, stream, L); break;
+        case kCtrlTable: hipLaunchKernelGGL((integrate<M, NR, kCtrlTable, W, STRICT>), grid, block, 0, stream, L); break;
+        case kCtrlConst: hipLaunchKernelGGL((integrate<M, NR, kCtrlConst, W, STRICT>), grid, block, 0, stream, L); break;
+        case kCtrlKnots: hipLaunchKernelGGL((integrate<M, NR, kCtrlKnots, W, STRICT>), grid, block, 0, stream, L); break;
+        case kCtrlSteps: hipLaunchKernelGGL((integrate<M, NR, kCtrlSteps, W, STRICT>), grid, block, 0, stream, L); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <class M, int W>
+template <class M, int W, bool STRICT = false>
 hipError_t launch_tire(const Launch& L, int block, hipStream_t stream) {
-    return L.nr ? launch_mode<M, true, W>(L, block, stream) : launch_mode<M, false, W>(L, block, stream);
+    return L.nr ? launch_mode<M, true, W, STRICT>(L, block, stream) : launch_mode<M, false, W, STRICT>(L, block, stream);
 }
 
 template <class M>
 hipError_t launch_model(const Launch& L, int block, hipStream_t stream) {
+    if (L.strict) return launch_tire<M, M::kWaves, true>(L, block, stream);
     const int w = L.waves == 0 ? M::kWaves : L.waves;
     if constexpr (M::kWaves >= 3) {
         if (w == 3) return launch_tire<M, 3>(L, block, stream);

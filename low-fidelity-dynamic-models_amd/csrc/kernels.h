@@ -43,6 +43,7 @@ struct Launch {
     const int* wave_hmax;
     int step_base;
     int waves;  // waves per SIMD of the kernel: 1 up to the model's maximum, or 0 for its default
+    int strict;  // 1: the step with the reference's own quotients and transcendentals (LFDM_ARITH_STRICT)
 };
 
 // Integration launches of the 11-, 18- and 24-DOF models (one kernel per tire model and control

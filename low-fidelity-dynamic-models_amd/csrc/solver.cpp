@@ -111,6 +111,7 @@ struct lfdm_solver {
     std::vector<int> horizon_host;
     bool horizon_dirty = false;
     int layout = LFDM_LAYOUT_GROUPED;
+    int arith = LFDM_ARITH_FAST;  // arithmetic of the integration step
     // device
     bool built = false;
     int S = 0, rec = 0;
@@ -454,6 +455,7 @@ Launch make_launch(lfdm_solver* h, int mode, double t0, int nsteps, double cs = 
     // a fleet of at most one wavefront per SIMD runs the kernels built for one wave per SIMD
     const int simds = simd_count(h);
     L.waves = (simds > 0 && h->S / kWave <= simds) ? 1 : 0;
+    L.strict = h->arith == LFDM_ARITH_STRICT ? 1 : 0;
     return L;
 }
 
@@ -1107,6 +1109,20 @@ int lfdm_set_layout(lfdm_solver* h, int layout) {
     if (!h->horizon_host.empty())
         if (int rc = drop_device_image(h)) return rc;
     h->layout = layout;
+    return LFDM_OK;
+}
+
+int lfdm_set_arithmetic(lfdm_solver* h, int arith) {
+    if (int rc = check(h)) return rc;
+    if (arith != LFDM_ARITH_FAST && arith != LFDM_ARITH_STRICT) return fail(LFDM_E_ARG, "unknown arithmetic");
+    h->arith = arith;
+    return LFDM_OK;
+}
+
+int lfdm_get_arithmetic(lfdm_solver* h, int* arith) {
+    if (int rc = check(h)) return rc;
+    if (!arith) return fail(LFDM_E_ARG, "null output");
+    *arith = h->arith;
     return LFDM_OK;
 }
 

@@ -14,6 +14,8 @@ LFDM_TIRE_TMEASY = 0
 LFDM_TIRE_TMEASYNR = 1
 LFDM_LAYOUT_GROUPED = 0
 LFDM_LAYOUT_BY_HORIZON = 1
+LFDM_ARITH_FAST = 0
+LFDM_ARITH_STRICT = 1
 ERRORS = {
     -1: "LFDM_E_ARG",
     -2: "LFDM_E_IO",
@@ -104,6 +106,8 @@ SIGNATURES = {
     "lfdm_set_horizons": [_p, _p],
     "lfdm_set_layout": [_p, _i],
     "lfdm_get_layout": [_p, ctypes.POINTER(_i)],
+    "lfdm_set_arithmetic": [_p, _i],
+    "lfdm_get_arithmetic": [_p, ctypes.POINTER(_i)],
     "lfdm_fleet_create": [ctypes.POINTER(_p)],
     "lfdm_fleet_destroy": [_p],
     "lfdm_fleet_add": [_p, _p],

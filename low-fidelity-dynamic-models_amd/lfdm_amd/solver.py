@@ -11,12 +11,15 @@ import ctypes
 
 import numpy as np
 
-from ._lib import LFDM_LAYOUT_BY_HORIZON, LFDM_LAYOUT_GROUPED, LFDM_TIRE_TMEASY, LFDM_TIRE_TMEASYNR, check, enc, lib
+from ._lib import (LFDM_ARITH_FAST, LFDM_ARITH_STRICT, LFDM_LAYOUT_BY_HORIZON, LFDM_LAYOUT_GROUPED, LFDM_TIRE_TMEASY,
+                   LFDM_TIRE_TMEASYNR, check, enc, lib)
 
 TMeasy = LFDM_TIRE_TMEASY
 TMeasyNr = LFDM_TIRE_TMEASYNR
 LAYOUT_GROUPED = LFDM_LAYOUT_GROUPED
 LAYOUT_BY_HORIZON = LFDM_LAYOUT_BY_HORIZON
+ARITH_FAST = LFDM_ARITH_FAST
+ARITH_STRICT = LFDM_ARITH_STRICT
 
 # Field names of the records (include/lfdm_amd.h).
 TIRE_FIELDS = ("xe", "ye", "xedot", "yedot", "omega", "dOmega", "xt", "rStat", "fx", "fy", "fz", "vsx", "vsy",
@@ -255,6 +258,16 @@ class SolverHalfImplicitGPU:
     def get_layout(self):
         v = ctypes.c_int()
         check(lib.lfdm_get_layout(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_arithmetic(self, arith):
+        """ARITH_FAST (default) or ARITH_STRICT: the step with the reference's own quotients,
+        atan2 / tan and hypot (see lfdm_set_arithmetic)."""
+        check(lib.lfdm_set_arithmetic(self._h, arith))
+
+    def get_arithmetic(self):
+        v = ctypes.c_int()
+        check(lib.lfdm_get_arithmetic(self._h, ctypes.byref(v)))
         return v.value
 
     # -- differentiation form (device pointers, e.g. torch tensors' data_ptr()) ---------------

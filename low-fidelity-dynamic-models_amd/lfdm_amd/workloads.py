@@ -11,6 +11,9 @@ Workloads (dt = 1e-3 s, HMMWV vehicle + TMeasy tire parameters, start at rest):
                 jittered x U(0.9, 1.1); throttle piecewise constant U(0, 1) per 1 s; steering
                 U(-0.3, 0.3) rad knots every 0.5 s, linearly interpolated; brake U(0, 0.5) on a
                 seeded 10 % of the 1 s intervals; a knot program per vehicle.
+  mixed_fleet -- 11/18/24-DOF, 2,000,000 vehicles split 40/40/20, per-vehicle horizons U(2, 20) s,
+                parameters jittered x U(0.85, 1.15), knot programs; 24-DOF vehicles start at
+                U(0, 20) m/s.
 """
 import os
 
@@ -132,6 +135,27 @@ def initial_speed_records(n, flat, dof=24, seed=7, umax=20.0, tire_nr=False):
             rec[:, i] = om
     rec[:, fields.index("crankOmega")] = om / gr0
     return rec
+
+
+def mixed_fleet(n_total, split=(0.4, 0.4, 0.2), tmin=2.0, tmax=20.0, jitter=0.15, seed=11, dt=1e-3):
+    """Per-model inputs of the mixed fleet (batching form), one model at a time: n_total vehicles split
+    over the 11-, 18- and 24-DOF HMMWV models, mass / yaw inertia / cornering stiffness jittered
+    x U(1 - jitter, 1 + jitter), knot-program controls over tmax, per-vehicle horizons
+    T_i ~ U(tmin, tmax) s (steps_i = round(T_i / dt)); 11/18-DOF vehicles start at rest, 24-DOF ones at
+    U(0, 20) m/s.  Yields dict(dof, n, flat, fl (knot_fleet), recs (24-DOF initial records or None),
+    horizon (int32 steps))."""
+    from . import params_from_json
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    for k, (dof, frac) in enumerate(zip((11, 18, 24), split)):
+        n = int(round(n_total * frac))
+        d = os.path.join(DATA, f"{dof}dof", "HMMWV")
+        flat = params_from_json(dof, f"{d}/vehicle.json", f"{d}/tmeasy.json",
+                                f"{d}/suspension.json" if dof == 24 else None)
+        fl = knot_fleet(n, flat, seed=seed + k, horizon=tmax, jitter=jitter, dof=dof)
+        recs = initial_speed_records(n, flat, dof, seed=seed + k) if dof == 24 else None
+        hz = np.rint(rng.uniform(tmin, tmax, n) / dt).astype(np.int32)
+        yield dict(dof=dof, n=n, flat=flat, fl=fl, recs=recs, horizon=hz)
 
 
 def knot_controls(t, ns, ds, np_, dp, knots):

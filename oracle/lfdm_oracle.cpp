@@ -5,6 +5,7 @@
 // runs vehicles independently (OpenMP over vehicles, one vehicle's whole horizon per task).
 #include <omp.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -122,10 +123,12 @@ struct M24 {
     template <class T> static void set(V<T>& s, const double* o) { d24::from_record(s, o); }
 };
 
+// horizon (optional): vehicle i takes min(nsteps, horizon[i]) steps; samples after its last step repeat
+// its final record.
 template <class M>
 int run_model(int nr, const double* flat, const double* ovr, const double* init, int n, double t0, double dt,
               int nsteps, int mode, const double* ctrl, const int* imeta, const double* dmeta, int every,
-              double* out, double* fin, int threads) {
+              double* out, double* fin, int threads, const int* horizon = nullptr) {
     const int rl = M::rec(nr != 0);
     Controls cs{mode, ctrl, imeta, dmeta, n};
     if (threads > 0) omp_set_num_threads(threads);
@@ -143,11 +146,14 @@ int run_model(int nr, const double* flat, const double* ovr, const double* init,
         }
         double t = t0;
         int rec = 0;
+        const int my = horizon ? std::min(nsteps, std::max(0, horizon[i])) : nsteps;
         for (int k = 0; k < nsteps; ++k) {
-            double st, th, br;
-            cs.at(i, k, t, st, th, br);
-            M::step(s, st, th, br, dt);
-            t = t + dt;
+            if (k < my) {
+                double st, th, br;
+                cs.at(i, k, t, st, th, br);
+                M::step(s, st, th, br, dt);
+                t = t + dt;
+            }
             if (out && every > 0 && (k + 1) % every == 0) {
                 M::get(s, out + ((size_t)rec * n + i) * rl);
                 ++rec;
@@ -181,6 +187,22 @@ int orc11_run(int nr, const double* flat, const double* ovr, const double* init,
               double* fin, int threads) {
     return run_model<M11>(nr, flat, ovr, init, n, t0, dt, nsteps, mode, ctrl, imeta, dmeta, every, out, fin,
                           threads);
+}
+
+// Same as orcNN_run for model `dof` (11, 18 or 24) with per-vehicle horizons horizon[n] (steps from t0).
+int orc_run_h(int dof, int nr, const double* flat, const double* ovr, const double* init, int n, double t0, double dt,
+              int nsteps, const int* horizon, int mode, const double* ctrl, const int* imeta, const double* dmeta,
+              int every, double* out, double* fin, int threads) {
+    if (dof == 11)
+        return run_model<M11>(nr, flat, ovr, init, n, t0, dt, nsteps, mode, ctrl, imeta, dmeta, every, out, fin,
+                              threads, horizon);
+    if (dof == 18)
+        return run_model<M18>(nr, flat, ovr, init, n, t0, dt, nsteps, mode, ctrl, imeta, dmeta, every, out, fin,
+                              threads, horizon);
+    if (dof == 24)
+        return run_model<M24>(nr, flat, ovr, init, n, t0, dt, nsteps, mode, ctrl, imeta, dmeta, every, out, fin,
+                              threads, horizon);
+    return -1;
 }
 
 int orc24_record_len(int nr) { return d24::record_len(nr != 0); }

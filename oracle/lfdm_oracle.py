@@ -37,6 +37,8 @@ def lib():
             getattr(L, f"orc{m}_record_len").restype = i
             getattr(L, f"orc{m}_run").argtypes = [i, p, p, p, i, d, d, i, i, p, p, p, i, p, p, i]
             getattr(L, f"orc{m}_run").restype = i
+        L.orc_run_h.argtypes = [i, i, p, p, p, i, d, d, i, p, i, p, p, p, i, p, p, i]
+        L.orc_run_h.restype = i
         L.orc24_init.argtypes = [i, p, p, i, p]
         L.orc24_init.restype = i
         L.orc18_count_ops.argtypes = [i, p, p, d, d, i, i, p, p, p, p]
@@ -98,11 +100,13 @@ def init24(flat, records, tire_nr=False, ovr=None):
     return recs
 
 
-def run(dof, flat, controls, n, nsteps, tire_nr=False, ovr=None, init=None, t0=0.0, dt=1e-3, every=0, threads=0):
+def run(dof, flat, controls, n, nsteps, tire_nr=False, ovr=None, init=None, t0=0.0, dt=1e-3, every=0, threads=0,
+        horizon=None):
     """Integrates n vehicles of one parameter set of the 11-, 18- or 24-DOF model.
 
     controls: ("steps", (nsteps, 3, n)) | ("table", rows (m, 4)) | ("knots", ns, ds, np, dp, (ns+2np, n)).
-    Returns (final records (n, rec), samples (nsteps // every, n, rec) or None)."""
+    horizon (optional, (n,) ints): vehicle i stops after horizon[i] steps; later samples repeat its final
+    record.  Returns (final records (n, rec), samples (nsteps // every, n, rec) or None)."""
     L = lib()
     rl = record_len(dof, tire_nr)
     flat = np.ascontiguousarray(flat, dtype=np.float64)
@@ -111,6 +115,12 @@ def run(dof, flat, controls, n, nsteps, tire_nr=False, ovr=None, init=None, t0=0
     init = None if init is None else np.ascontiguousarray(np.broadcast_to(init, (n, rl)), dtype=np.float64)
     fin = np.zeros((n, rl))
     out = np.zeros((nsteps // every, n, rl)) if every > 0 else None
+    if horizon is not None:
+        hz = np.ascontiguousarray(horizon, dtype=np.int32)
+        assert hz.shape == (n,)
+        assert L.orc_run_h(dof, int(tire_nr), _ptr(flat), _ptr(ovr), _ptr(init), n, t0, dt, nsteps, _ptr(hz), mode,
+                           _ptr(ctrl), _ptr(im), _ptr(dm), every, _ptr(out), _ptr(fin), threads) == 0
+        return fin, out
     getattr(L, f"orc{dof}_run")(int(tire_nr), _ptr(flat), _ptr(ovr), _ptr(init), n, t0, dt, nsteps, mode,
                                  _ptr(ctrl), _ptr(im), _ptr(dm), every, _ptr(out), _ptr(fin), threads)
     return fin, out

@@ -125,7 +125,7 @@ void init_tire_sus(Vehicle<T>& s) {
 
 // vehToSusTransform (dof24.cpp:148-230)
 template <class T>
-void veh_to_sus(Vehicle<T>& s) {
+ORC_REF_FN void veh_to_sus(Vehicle<T>& s) {
     const auto& p = s.vp;
     const auto& v = s.vs;
     SusState<T>* q = s.sus;
@@ -160,7 +160,7 @@ void veh_to_sus(Vehicle<T>& s) {
 
 // vehToTireTransform (dof24.cpp:313-385); reads the previous step's rStat
 template <class T>
-void veh_to_tire(Vehicle<T>& s, const T& steering) {
+ORC_REF_FN void veh_to_tire(Vehicle<T>& s, const T& steering) {
     using std::cos;
     using std::sin;
     const auto& v = s.vs;
@@ -187,7 +187,7 @@ void veh_to_tire(Vehicle<T>& s, const T& steering) {
 // computeTireRHS (dof24.cpp:827-987, Nr :988-1121): fz from the compression state, rStat from the
 // attitude, then the 18-DOF force core.
 template <class T>
-void tire_rhs(Vehicle<T>& s, TireState<T>& ts, const T& steering) {
+ORC_REF_FN void tire_rhs(Vehicle<T>& s, TireState<T>& ts, const T& steering) {
     using std::cos;
     const auto& v = s.vs;
     const auto& tp = s.tp;
@@ -217,7 +217,7 @@ void tire_rhs(Vehicle<T>& s, TireState<T>& ts, const T& steering) {
 
 // computeTireCompressionVelocity (dof24.cpp:1122-1151)
 template <class T>
-void tire_compression(Vehicle<T>& s) {
+ORC_REF_FN void tire_compression(Vehicle<T>& s) {
     using std::cos;
     using std::sin;
     const auto& v = s.vs;
@@ -229,7 +229,7 @@ void tire_compression(Vehicle<T>& s) {
 
 // tireToVehTransform (dof24.cpp:459-528)
 template <class T>
-void tire_to_veh(Vehicle<T>& s, const T& steering) {
+ORC_REF_FN void tire_to_veh(Vehicle<T>& s, const T& steering) {
     using std::cos;
     using std::sin;
     const auto& v = s.vs;
@@ -251,7 +251,7 @@ void tire_to_veh(Vehicle<T>& s, const T& steering) {
 
 // computeSusRHS (dof24.cpp:1182-1229): reads last step's fzd and dxs, then updates dxs
 template <class T>
-void sus_rhs(Vehicle<T>& s) {
+ORC_REF_FN void sus_rhs(Vehicle<T>& s) {
     using std::cos;
     using std::sin;
     const auto& v = s.vs;
@@ -270,7 +270,7 @@ void sus_rhs(Vehicle<T>& s) {
 // computeForcesThroughSus (dof24.cpp:599-680), including the rear-right lateral force that the
 // reference writes without the cos(theta) factor (:639)
 template <class T>
-void forces_through_sus(Vehicle<T>& s) {
+ORC_REF_FN void forces_through_sus(Vehicle<T>& s) {
     using std::cos;
     using std::sin;
     const auto& v = s.vs;
@@ -306,7 +306,7 @@ void forces_through_sus(Vehicle<T>& s) {
 
 // computeVehicleRHS (dof24.cpp:1682-1733)
 template <class T>
-void veh_rhs(Vehicle<T>& s) {
+ORC_REF_FN void veh_rhs(Vehicle<T>& s) {
     using std::cos;
     using std::sin;
     using std::tan;

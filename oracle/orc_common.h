@@ -19,6 +19,14 @@ static const double kG = 9.81;
 
 inline double val(double x) { return x; }
 
+// Marks a restated function that is its own function in the reference's physics sources
+// (dof24.cpp).  g++ -O2 and above turns sin(x) and cos(x) of one argument inside one function into a
+// single sincos call, and glibc's sincos does not round exactly like its sin and cos for every
+// argument; the reference's physics functions are compiled apart from the step that calls them, so
+// each pairs only its own calls.  Keeping these functions out of line gives the restatement the same
+// pairing (the step's Euler-angle rates, for one, call cos(theta) without sin(theta)).
+#define ORC_REF_FN __attribute__((noinline))
+
 template <class T>
 struct MapE {
     T x, y;

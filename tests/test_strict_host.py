@@ -1,0 +1,88 @@
+"""The solver's step code compiled for the host (tests/cpp/host_step.cpp) in both arithmetic forms,
+against the oracle.
+
+On the host every quotient is a division and every transcendental glibc's, so the strict form
+(LFDM_ARITH_STRICT) must reproduce the oracle, and with it the reference CPU solver, bit for bit: that
+pins the strict step's formulas (the reference's quotients, atan2 / tan slip angle, hypot, (dx num) /
+den, and which sine / cosine pairs the reference's compiled functions take from one sincos) without a
+GPU.  The fast form's reformulations (tangent-difference slip angle, sqrt for hypot, reciprocals of
+shared denominators) change only rounding, so the fast form is held to the fleet parity bar of
+fleetcheck.py: 1e-9 for vehicles whose trajectory does not respond to a one-ulp parameter change, the
+model's own one-ulp response (times fleetcheck.SPREAD_MULT) for the others.  These fleets include
+ill-conditioned vehicles (the Sedan's 24-DOF TMeasyNr vehicles started at speed pitch and roll hard).
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import DATA, ROOT
+
+import fleetcheck as F
+import lfdm_amd
+import lfdm_oracle as O
+from lfdm_amd import workloads as W
+
+LIB = os.path.join(ROOT, "low-fidelity-dynamic-models_amd", "lfdm_amd", "libhost_step.so")
+
+CASES = [  # dof, vehicle, tire, TMeasyNr?, suspension
+    (18, "HMMWV/vehicle.json", "HMMWV/tmeasy.json", False, None),
+    (18, "Sedan/vehicle.json", "Sedan/tmeasyNr.json", True, None),
+    (18, "ART/vehicle.json", "ART/tmeasy.json", False, None),
+    (11, "HMMWV/vehicle.json", "HMMWV/tmeasy.json", False, None),
+    (11, "Sedan/vehicle.json", "Sedan/tmeasyNr.json", True, None),
+    (24, "HMMWV/vehicle.json", "HMMWV/tmeasy.json", False, "HMMWV/suspension.json"),
+    (24, "Sedan/vehicle.json", "Sedan/tmeasyNr.json", True, "Sedan/suspension.json"),
+]
+
+
+def host_lib():
+    assert os.path.exists(LIB), "run make -C low-fidelity-dynamic-models_amd hoststep"
+    L = ctypes.CDLL(LIB)
+    i, p, d = ctypes.c_int, ctypes.c_void_p, ctypes.c_double
+    L.host_step_run.argtypes = [i, i, i, p, i, p, p, i, i, i, p, d, p, ctypes.c_char_p, i]
+    L.host_step_run.restype = i
+    return L
+
+
+def host_run(L, dof, nr, strict, flat, init, ctrl, ovr):
+    n = ctrl.shape[2]
+    fin = np.zeros_like(init)
+    err = ctypes.create_string_buffer(256)
+    rc = L.host_step_run(dof, int(nr), int(strict), flat.ctypes.data, flat.size, ovr.ctypes.data, init.ctypes.data, n,
+                         init.shape[1], ctrl.shape[0], ctrl.ctypes.data, 1e-3, fin.ctypes.data, err, 256)
+    assert rc == 0, err.value
+    return fin
+
+
+def host_sampled(L, dof, nr, strict, flat, init, ctrl, ovr, every):
+    """Records after every `every` steps (chunked runs continue from the records, as launches do)."""
+    out, cur = [], init
+    for k in range(ctrl.shape[0] // every):
+        cur = host_run(L, dof, nr, strict, flat, cur, np.ascontiguousarray(ctrl[k * every:(k + 1) * every]), ovr)
+        out.append(cur)
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("dof,veh,tire,nr,sus", CASES, ids=[f"{c[0]}-{c[1].split('/')[0]}-{c[2]}" for c in CASES])
+def test_host_step_forms_vs_oracle(dof, veh, tire, nr, sus):
+    L = host_lib()
+    d = os.path.join(DATA, f"{dof}dof")
+    flat = lfdm_amd.params_from_json(dof, f"{d}/{veh}", f"{d}/{tire}", f"{d}/{sus}" if sus else None, int(nr))
+    n, nsteps, every = 48, 10000, 500
+    # steering 0.1 sin(...) rad through a linear map of maxSteer 0.3
+    ctrl = np.ascontiguousarray(W.ramp_sine(n, nsteps=nsteps, seed=3, max_steer=0.3))
+    ovr = W.knot_fleet(n, flat, seed=4, dof=dof, tire_nr=nr)["ovr"]
+    recs = W.initial_speed_records(n, flat, dof, seed=6, tire_nr=nr) if dof == 24 else None
+    case = dict(dof=dof, nr=nr, n=n, flat=flat, fl=dict(ovr=ovr), ctrl=("steps", ctrl), recs=recs)
+    init = F.oracle_init(O, case, ovr)
+    init = np.zeros((n, O.record_len(dof, nr))) if init is None else init
+    ref, upto, spread = F.reference(O, case, nsteps, every)
+    strict = host_sampled(L, dof, nr, True, flat, init, ctrl, ovr, every)
+    np.testing.assert_array_equal(strict, ref)
+    fast = host_sampled(L, dof, nr, False, flat, init, ctrl, ovr, every)
+    v = F.verdict(fast, ref, upto, spread, F.SPREAD_MULT)
+    print(F.summary_line(f"{dof}-DOF {veh} {tire}, fast form", v))
+    assert v["ok"], v["failing"]
+    assert not np.array_equal(fast, ref)
