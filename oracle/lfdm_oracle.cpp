@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "orc_count.h"
+#include "orc_dual.h"
 #include "orc11.h"
 #include "orc18.h"
 #include "orc24.h"
@@ -225,6 +226,85 @@ int orc24_init(int nr, const double* flat, const double* ovr, int n, double* rec
         d24::to_record(s, recs + (size_t)i * rl);
     }
     return 0;
+}
+
+// Forward-mode tangents of the tracking loss L = sum_k dt ((x_k - xr_k)^2 + (y_k - yr_k)^2) of n 18-DOF
+// vehicles, (x_k, y_k) the position after step k (the integrand of the reference's FSA tracking cost,
+// dof18_sundials.cpp:868-884), over nsteps steps of per-step controls ctrl[nsteps][3][n] from the records
+// init[n][rec] with per-vehicle parameters ovr[7][n] (m, jz, jx, dfy0Pn, dfy0P2n, torque scale,
+// maxBrakeTorque), xyref[nsteps][2][n].  Three tangent directions per vehicle: 0 along dctrl (layout of
+// ctrl), 1 along dovr (layout of ovr), 2 along dinit (layout of init).  Writes loss[n] and
+// dloss[3][n] (the directional derivatives).
+int orc18_loss_tangents(int nr, const double* flat, const double* ovr, const double* init, int n, int nsteps,
+                        double dt, const double* ctrl, const double* xyref, const double* dctrl, const double* dovr,
+                        const double* dinit, double* loss, double* dloss, int threads) {
+    using D = Dual<3>;
+    const int rl = d18::record_len(nr != 0);
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < n; ++i) {
+        d18::Vehicle<D> s;
+        s.nr = nr != 0;
+        d18::read_params(flat, s.nr, s.vp, s.tp);
+        auto par = [&](int f) {
+            D x(ovr[(size_t)f * n + i]);
+            x.d[1] = dovr[(size_t)f * n + i];
+            return x;
+        };
+        s.vp.m = par(0);
+        s.vp.jz = par(1);
+        s.vp.jx = par(2);
+        s.tp.dfy0Pn = par(3);
+        s.tp.dfy0P2n = par(4);
+        const D sc = par(5);
+        for (auto& e : s.vp.torque) e.y = e.y * sc;
+        s.vp.maxBrakeTorque = par(6);
+        d18::from_record_src(s, [&](int k) {
+            D x(init[(size_t)i * rl + k]);
+            x.d[2] = dinit[(size_t)i * rl + k];
+            return x;
+        });
+        D L(0.0);
+        for (int k = 0; k < nsteps; ++k) {
+            D c[3];
+            for (int q = 0; q < 3; ++q) {
+                c[q] = D(ctrl[((size_t)k * 3 + q) * n + i]);
+                c[q].d[0] = dctrl[((size_t)k * 3 + q) * n + i];
+            }
+            d18::step(s, c[0], c[1], c[2], dt);
+            const D ex = s.vs.x - xyref[((size_t)k * 2 + 0) * n + i];
+            const D ey = s.vs.y - xyref[((size_t)k * 2 + 1) * n + i];
+            L = L + dt * (ex * ex + ey * ey);
+        }
+        loss[i] = L.v;
+        for (int j = 0; j < 3; ++j) dloss[(size_t)j * n + i] = L.d[j];
+    }
+    return 0;
+}
+
+// Tangents of one operation of the dual scalar at x (and y for the two-argument ones), for the unit
+// tests of its derivative rules: op 0 sin, 1 cos, 2 tan, 3 tanh, 4 sqrt, 5 fabs, 6 atan2, 7 hypot,
+// 8 x / y, 9 x * y.  Returns the value; *dx, *dy receive the partial derivatives.
+double orc_dual_op(int op, double x, double y, double* dx, double* dy) {
+    using D = Dual<2>;
+    D a(x), b(y), r;
+    a.d[0] = 1.0;
+    b.d[1] = 1.0;
+    switch (op) {
+        case 0: r = sin(a); break;
+        case 1: r = cos(a); break;
+        case 2: r = tan(a); break;
+        case 3: r = tanh(a); break;
+        case 4: r = sqrt(a); break;
+        case 5: r = fabs(a); break;
+        case 6: r = atan2(a, b); break;
+        case 7: r = hypot(a, b); break;
+        case 8: r = a / b; break;
+        default: r = a * b; break;
+    }
+    *dx = r.d[0];
+    *dy = r.d[1];
+    return r.v;
 }
 
 // Counts operations over nsteps of one vehicle (same arguments as orc18_run with n = 1).

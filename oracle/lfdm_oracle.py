@@ -39,6 +39,10 @@ def lib():
             getattr(L, f"orc{m}_run").restype = i
         L.orc_run_h.argtypes = [i, i, p, p, p, i, d, d, i, p, i, p, p, p, i, p, p, i]
         L.orc_run_h.restype = i
+        L.orc18_loss_tangents.argtypes = [i, p, p, p, i, i, d, p, p, p, p, p, p, p, i]
+        L.orc18_loss_tangents.restype = i
+        L.orc_dual_op.argtypes = [i, d, d, p, p]
+        L.orc_dual_op.restype = d
         L.orc24_init.argtypes = [i, p, p, i, p]
         L.orc24_init.restype = i
         L.orc18_count_ops.argtypes = [i, p, p, d, d, i, i, p, p, p, p]
@@ -124,6 +128,29 @@ def run(dof, flat, controls, n, nsteps, tire_nr=False, ovr=None, init=None, t0=0
     getattr(L, f"orc{dof}_run")(int(tire_nr), _ptr(flat), _ptr(ovr), _ptr(init), n, t0, dt, nsteps, mode,
                                  _ptr(ctrl), _ptr(im), _ptr(dm), every, _ptr(out), _ptr(fin), threads)
     return fin, out
+
+
+def loss_tangents18(flat, ovr, init, ctrl, xyref, dctrl, dovr, dinit, dt=1e-3, tire_nr=False, threads=0):
+    """Tracking loss of n 18-DOF vehicles and its forward-mode directional derivatives (dual-number
+    oracle): ctrl / dctrl (T, 3, n), ovr / dovr (7, n), init / dinit (n, rec), xyref (T, 2, n).  Returns
+    (loss (n,), dloss (3, n): along dctrl, dovr, dinit)."""
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    flat, ovr, init, ctrl, xyref, dctrl, dovr, dinit = map(c, (flat, ovr, init, ctrl, xyref, dctrl, dovr, dinit))
+    T, _, n = ctrl.shape
+    assert xyref.shape == (T, 2, n) and dctrl.shape == ctrl.shape and dovr.shape == ovr.shape == (7, n)
+    assert init.shape == dinit.shape == (n, record_len(18, tire_nr))
+    loss, dloss = np.zeros(n), np.zeros((3, n))
+    assert lib().orc18_loss_tangents(int(tire_nr), _ptr(flat), _ptr(ovr), _ptr(init), n, T, dt, _ptr(ctrl),
+                                     _ptr(xyref), _ptr(dctrl), _ptr(dovr), _ptr(dinit), _ptr(loss), _ptr(dloss),
+                                     threads) == 0
+    return loss, dloss
+
+
+def dual_op(op, x, y=0.0):
+    """(value, d/dx, d/dy) of one dual-scalar operation (see orc_dual_op)."""
+    dx, dy = ctypes.c_double(), ctypes.c_double()
+    v = lib().orc_dual_op(op, x, y, ctypes.byref(dx), ctypes.byref(dy))
+    return v, dx.value, dy.value
 
 
 OP_NAMES = ("add", "mul", "div", "sqrt", "sin", "cos", "tan", "atan2", "hypot", "tanh")

@@ -595,25 +595,32 @@ void to_record(const Vehicle<T>& s, double* o) {
     o[k++] = (double)v.gear;
 }
 
-template <class T>
-void from_record(Vehicle<T>& s, const double* o) {
+// Reads a record through src(k), which returns record entry k as a T (a dual scalar seeds its
+// tangents there).
+template <class T, class Src>
+void from_record_src(Vehicle<T>& s, Src&& src) {
     int k = 0;
     for (int i = 0; i < 4; ++i) {
         TireState<T>& t = s.tire[i];
         if (!s.nr) {
-            t.xe = T(o[k++]); t.ye = T(o[k++]); t.xedot = T(o[k++]); t.yedot = T(o[k++]);
+            t.xe = src(k++); t.ye = src(k++); t.xedot = src(k++); t.yedot = src(k++);
         } else {
             t.xe = t.ye = t.xedot = t.yedot = T(0.0);
         }
-        t.omega = T(o[k++]); t.dOmega = T(o[k++]); t.xt = T(o[k++]); t.rStat = T(o[k++]);
-        t.fx = T(o[k++]); t.fy = T(o[k++]); t.fz = T(o[k++]); t.vsx = T(o[k++]); t.vsy = T(o[k++]);
-        t.My = T(o[k++]); t.engTor = T(o[k++]);
+        t.omega = src(k++); t.dOmega = src(k++); t.xt = src(k++); t.rStat = src(k++);
+        t.fx = src(k++); t.fy = src(k++); t.fz = src(k++); t.vsx = src(k++); t.vsy = src(k++);
+        t.My = src(k++); t.engTor = src(k++);
     }
     VehState<T>& v = s.vs;
     T* f[] = {&v.x, &v.y, &v.u, &v.v, &v.psi, &v.wz, &v.phi, &v.wx, &v.udot, &v.vdot, &v.wxdot, &v.wzdot,
               &v.crankOmega, &v.dOmega_crank};
-    for (T* p : f) *p = T(o[k++]);
-    v.gear = (int)o[k++];
+    for (T* p : f) *p = src(k++);
+    v.gear = (int)val(src(k++));
+}
+
+template <class T>
+void from_record(Vehicle<T>& s, const double* o) {
+    from_record_src(s, [&](int k) { return T(o[k]); });
 }
 
 }  // namespace d18

@@ -2,7 +2,9 @@
 (csrc/d18_adjoint.h, kernels in csrc/d18_adjoint.hip) against fourth-order central finite
 differences of the GPU forward map, through the C ABI and through torch.autograd.
 
-Bar: for every input group (controls, per-vehicle parameters scaled by their values, initial state)
+The oracle pin: test_adjoint_matches_dual_oracle_on_tracking_workload compares the adjoint with the
+oracle's forward-mode tangents (<= 1e-8 relative) on the tracking workload.  The finite-difference
+check: for every input group (controls, per-vehicle parameters scaled by their values, initial state)
 max |adjoint - fd| <= 1e-6 max |fd| (normwise relative error of the gradient); 1e-4 for the tire
 deflections xe, ye, whose finite differences stop converging near 1e-5 (they are ~1e-5 m and stiff:
 steps small enough to stay off the slip-force kinks come within a decade of the rounding noise;
@@ -136,6 +138,95 @@ def test_adjoint_matches_fourth_order_fd():
     # identical vehicles get identical gradients; untouched record fields get none
     assert np.all(gs[fields.index("lf.fx"), :] == 0.0)
     s.close()
+
+
+def tracking_case(n, T, seed=21):
+    """The differentiation-form workload at test size: HMMWV vehicles starting straight at 15 m/s,
+    tracking a seeded arc (curvature U(-0.02, 0.02) 1/m at 15 m/s) with the loss of lfdm_amd.mpc; the
+    per-vehicle parameters jittered x U(0.9, 1.1); controls the MPC's feed-forward guess plus smooth
+    per-vehicle variations (steering, throttle, braking pulses), so gear, brake and slip states vary."""
+    from lfdm_amd.mpc import arc_reference, initial_controls, moving_start
+
+    flat = hmmwv()
+    ref, kappa = arc_reference(n, T, seed=seed, device="cpu")
+    rng = np.random.default_rng(seed)
+    ovr = W.default_overrides(flat, n)
+    ovr *= rng.uniform(0.9, 1.1, (7, 1)) * np.ones((1, n))
+    ovr[:, :] *= rng.uniform(0.95, 1.05, (7, n))
+    init = moving_start(flat, n).numpy()
+    ctrl = initial_controls(flat, n, T, kappa, device="cpu").numpy()
+    t = np.arange(T) * 1e-3
+    ph = rng.uniform(0, 2 * np.pi, (3, n))
+    ctrl[:, 0, :] += 0.05 * np.sin(1.3 * t[:, None] + ph[0])
+    ctrl[:, 1, :] += 0.2 * np.sin(0.9 * t[:, None] + ph[1])
+    ctrl[:, 2, :] = np.where(np.sin(1.7 * t[:, None] + ph[2]) > 0.8, 0.05, 0.0)
+    return flat, ovr, init, np.ascontiguousarray(ctrl), ref.numpy()
+
+
+def test_adjoint_matches_dual_oracle_on_tracking_workload():
+    """The GPU adjoint's gradients of the arc tracking loss (5,000 steps, checkpoints every 100 steps,
+    512 vehicles) against the oracle's forward-mode tangents (dual-number restatement of the reference
+    step, oracle/orc_dual.h, pinned in tests/test_oracle_dual.py): per vehicle, the directional
+    derivative along a random direction over the 15,000 controls, one over the 7 per-vehicle parameters
+    and one over the initial record (tire deflections included), relative difference <= 1e-8.
+
+    Both are exact derivatives of the piece of the discrete map the trajectory is on; the GPU's
+    trajectory differs from the oracle's by device-libm rounding, so a vehicle whose derivative itself
+    moves when a parameter moves by one ulp (the oracle's derivative spread; an event such as a gear
+    shift lands on another step) is held to 25 x that spread instead, like the fleet parity tests."""
+    import lfdm_oracle as O
+
+    n, T, K, dt = 512, 5000, 100, 1e-3
+    flat, ovr, init, ctrl, ref = tracking_case(n, T)
+    rng = np.random.default_rng(8)
+    fields = lfdm_amd.record_fields(18)
+    dctrl = rng.standard_normal(ctrl.shape)
+    dovr = rng.standard_normal(ovr.shape) * np.abs(ovr)
+    scale = np.abs(init) + 1e-3
+    scale[:, [i for i, f in enumerate(fields) if f.endswith((".xe", ".ye"))]] = 1e-5
+    dinit = rng.standard_normal(init.shape) * scale
+    say(f"[adjoint] tracking workload: {n} vehicles x {T} steps, K = {K}: GPU forward + backward")
+    s = lfdm_amd.SolverHalfImplicitGPU(18, n)
+    s.construct_flat(flat, n, None, lfdm_amd.TMeasy)
+    s.Initialize(s.zero_record(), n)
+    s.set_overrides(ovr)
+    s.set_states(init)
+    dc = torch.from_numpy(ctrl).cuda()
+    xy = torch.empty((T, 2, n), dtype=torch.float64, device="cuda")
+    s.adjoint_forward(T, K, dc.data_ptr(), xy.data_ptr())
+    s.synchronize()
+    e = xy.cpu().numpy() - ref
+    loss_gpu = dt * (e * e).sum(axis=(0, 1))
+    xyb = torch.from_numpy(np.ascontiguousarray(2.0 * dt * e)).cuda()
+    gc = torch.empty_like(dc)
+    gs = torch.empty((len(fields), n), dtype=torch.float64, device="cuda")
+    gp = torch.empty((7, n), dtype=torch.float64, device="cuda")
+    s.adjoint_backward(xyb.data_ptr(), gc.data_ptr(), gs.data_ptr(), gp.data_ptr())
+    s.synchronize()
+    s.close()
+    gc, gs, gp = gc.cpu().numpy(), gs.cpu().numpy(), gp.cpu().numpy()
+    gpu = np.stack([(gc * dctrl).sum(axis=(0, 1)), (gp * dovr).sum(axis=0), (gs.T * dinit).sum(axis=1)])
+    say("[adjoint] oracle tangents and their one-ulp spread")
+    loss, dl = O.loss_tangents18(flat, ovr, init, ctrl, ref, dctrl, dovr, dinit)
+    spread = np.zeros((3, n))
+    for f in (0, 1, 2, 3, 5, 6):
+        for direction in (np.inf, -np.inf):
+            o2 = ovr.copy()
+            o2[f] = np.nextafter(o2[f], direction)
+            _, d2 = O.loss_tangents18(flat, o2, init, ctrl, ref, dctrl, dovr, dinit)
+            spread = np.maximum(spread, np.abs(d2 - dl) / np.abs(dl))
+    assert np.isfinite(gpu).all() and np.isfinite(dl).all()
+    rel = np.abs(gpu - dl) / np.abs(dl)
+    stable = spread <= 1e-10
+    bound = np.where(stable, 1e-8, np.maximum(1e-8, 25.0 * spread))
+    lrel = np.abs(loss_gpu - loss) / loss
+    for j, name in enumerate(("controls", "parameters", "initial record")):
+        say(f"[adjoint] {name}: {stable[j].sum()} of {n} vehicles derivative-stable, max rel {rel[j][stable[j]].max():.2e}"
+            f" (median {np.median(rel[j]):.1e}); others max rel / spread "
+            f"{(rel[j][~stable[j]] / spread[j][~stable[j]]).max() if (~stable[j]).any() else 0:.2g}")
+    say(f"[adjoint] loss: max rel difference {lrel.max():.2e}")
+    assert (rel <= bound).all(), [np.flatnonzero(~(rel[j] <= bound[j]))[:10].tolist() for j in range(3)]
+    assert stable.mean() > 0.9
 
 
 def test_torch_rollout_backward_and_mpc_step():

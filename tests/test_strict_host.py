@@ -78,11 +78,11 @@ def test_host_step_forms_vs_oracle(dof, veh, tire, nr, sus):
     case = dict(dof=dof, nr=nr, n=n, flat=flat, fl=dict(ovr=ovr), ctrl=("steps", ctrl), recs=recs)
     init = F.oracle_init(O, case, ovr)
     init = np.zeros((n, O.record_len(dof, nr))) if init is None else init
-    ref, upto, spread = F.reference(O, case, nsteps, every)
+    R = F.reference(O, case, nsteps, every)
     strict = host_sampled(L, dof, nr, True, flat, init, ctrl, ovr, every)
-    np.testing.assert_array_equal(strict, ref)
+    np.testing.assert_array_equal(strict, R.traj)
     fast = host_sampled(L, dof, nr, False, flat, init, ctrl, ovr, every)
-    v = F.verdict(fast, ref, upto, spread, F.SPREAD_MULT)
+    v = F.verdict(fast, R)
     print(F.summary_line(f"{dof}-DOF {veh} {tire}, fast form", v))
     assert v["ok"], v["failing"]
-    assert not np.array_equal(fast, ref)
+    assert not np.array_equal(fast, R.traj)
