@@ -248,6 +248,23 @@ int lfdm_fleet_synchronize(lfdm_fleet* f);
 /* Device time of the last lfdm_fleet_run, from HIP events around the graph launch. */
 int lfdm_fleet_last_ms(lfdm_fleet* f, double* ms);
 
+/* ---- no-allocation entry point (no counterpart in the reference) ---------------------------
+ * For callers that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc): integrates n vehicles of one parameter set (the flat vector of
+ * lfdm_params_from_json) nsteps steps of dt with per-step controls device_ctrl[nsteps][3][n] from the
+ * records device_records_in[n][record_len] (taken as they are, like lfdm_set_states: no 24-DOF
+ * initializeTireSus) with per-vehicle parameters device_ovr[7][n] (order of lfdm_set_overrides; NULL: the
+ * set's own values), and writes the final records to device_records_out[n][record_len] (may alias
+ * device_records_in).  The same kernels and results as a handle doing Initialize / set_states,
+ * set_overrides, set_controls_steps_device, advance and get_states.  Asynchronous on `stream` (hipStream_t,
+ * NULL = the default stream); calls no hipMalloc / hipFree: the parameter set, slot maps, SoA state image
+ * and per-vehicle parameters live in a 64 MiB device array of the library, one per GPU, held until the
+ * process ends.  Sizes beyond it (about 100,000 18-DOF, 190,000 11-DOF or 37,000 24-DOF vehicles) fail
+ * with LFDM_E_CAPACITY.  Calls that overlap on one GPU share that array and race. */
+int lfdm_integrate_noalloc(int dof, const double* flat, int len, int tire_type, unsigned n,
+                           const double* device_records_in, const double* device_ovr, const double* device_ctrl,
+                           int nsteps, double dt, int arith, double* device_records_out, void* stream);
+
 /* Stream used for every launch and copy from now on (hipStream_t).  NULL is the default stream,
  * which is ordered with every blocking stream (torch's default stream is NULL).  A handle whose
  * caller never chose a stream uses a non-blocking stream of its own. */

@@ -101,6 +101,9 @@ hipError_t launch_adj_reverse(const AdjLaunch& L, hipStream_t stream);
 // Every vehicle's step Jacobian at its current record under the per-vehicle controls ctrl[3][n_user].
 hipError_t launch_step_jacobian(const AdjLaunch& L, hipStream_t stream);
 
+// Records `msg` as the calling thread's lfdm_last_error() text and returns `code`.
+int set_last_error(int code, const char* msg);
+
 // Gathers full records of `n` slots from SoA state into out[n][rec].
 hipError_t launch_gather(const double* state, int S, int rec, const int* slots, int n, double* out,
                          hipStream_t stream);

@@ -34,6 +34,13 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+// Sets the calling thread's last-error text from other translation units of the library.
+int lfdm::set_last_error(int code, const char* msg) { return fail(code, msg); }
+
+namespace {
+
 #define HIPCHK(call)                                                                           \
     do {                                                                                       \
         hipError_t e_ = (call);                                                                \

@@ -119,6 +119,7 @@ SIGNATURES = {
     "lfdm_fleet_run": [_p, _i],
     "lfdm_fleet_synchronize": [_p],
     "lfdm_fleet_last_ms": [_p, _pd],
+    "lfdm_integrate_noalloc": [_i, _p, _i, _i, _u, _p, _p, _p, _i, _d, _i, _p, _p],
 }
 
 for _name, _args in SIGNATURES.items():

@@ -169,3 +169,20 @@ def test_initialize24_runs_tire_suspension_init(name, golden24):
     s.Initialize(rec, n)
     np.testing.assert_array_equal(s.get_states(), init)
     s.close()
+
+
+def test_noalloc_refuses_fleets_beyond_its_array():
+    """lfdm_integrate_noalloc checks the fleet against its fixed device array before any GPU work (no GPU
+    is touched here: the pointers are never read)."""
+    import ctypes
+
+    from lfdm_amd._lib import lib
+
+    flat = lfdm_amd.params_from_json(18, f"{DATA}/18dof/HMMWV/vehicle.json", f"{DATA}/18dof/HMMWV/tmeasy.json")
+    rc = lib.lfdm_integrate_noalloc(18, flat.ctypes.data, flat.size, lfdm_amd.TMeasy, 200_000, 8, None, 8, 10, 1e-3,
+                                    0, 8, ctypes.c_void_p(0))
+    assert rc == -5
+    assert "at most 64 MiB" in lib.lfdm_last_error().decode()
+    rc = lib.lfdm_integrate_noalloc(18, flat.ctypes.data, flat.size, lfdm_amd.TMeasy, 0, 8, None, 8, 10, 1e-3, 0, 8,
+                                    ctypes.c_void_p(0))
+    assert rc == -1
