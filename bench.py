@@ -18,8 +18,10 @@ workload, with the oracle (the CPU restatement, bit-identical to the reference s
 vehicles) beside it under `port`; without oracle/_ref the oracle is the baseline.  The default
 18-DOF line also carries `other_models` (the 24-DOF configuration, 500,000 vehicles over the same
 horizon, and the 11-DOF model on the headline fleet), `adjoint` (the differentiation form on 65,536
-vehicles x 5,000 steps, checkpoints every 100, with 20 MPC iterations) and `mixed_fleet` (the
-batching form on 2,000,000 vehicles of the three models with horizons U(2, 20) s).
+vehicles x 5,000 steps, checkpoints every 100, with 20 MPC iterations; its headline
+`adjoint_forward_cost_ratio` is (forward + backward) / plain forward at the named tape budget),
+`mixed_fleet` (the batching form on 2,000,000 vehicles of the three models with horizons U(2, 20) s)
+and `n_sweep` (the reference paper's fleet-size sweep on the acc3 driver file, with its A100 times).
 """
 import argparse
 import json
@@ -38,22 +40,25 @@ HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 
 def flops_per_step(dof, flat, nsteps=10000, nveh=4, seed=2):
     """Algorithmic FLOPs per vehicle-step: + - * / sqrt count 1, sin/cos/tan/atan2/tanh count 20,
-    hypot counts 4 (oracle counting scalar over the knot_fleet workload)."""
+    hypot counts 4 (oracle counting scalar over the knot_fleet workload).  Returns (with every libm
+    call the reference writes, without libm weights, with each step's distinct libm calls only: a call
+    repeating one of the same step on the same argument counted once)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import lfdm_oracle as O
     from lfdm_amd import workloads as W
 
     fl = W.knot_fleet(nveh, flat, seed=seed, dof=dof)
     init = initial_records(O, dof, flat, nveh, fl["ovr"])
-    basic = trans = 0
+    basic = trans = dtrans = 0
     for i in range(nveh):
         knots = fl["knots"][:, i:i + 1]
         c = O.count_ops(dof, flat, ("knots", fl["ns"], fl["ds"], fl["np"], fl["dp"], knots), nsteps,
                         init=None if init is None else init[i])
         basic += c["add"] + c["mul"] + c["div"] + c["sqrt"]
         trans += 20 * (c["sin"] + c["cos"] + c["tan"] + c["atan2"] + c["tanh"]) + 4 * c["hypot"]
+        dtrans += 20 * sum(c[f + "_distinct"] for f in ("sin", "cos", "tan", "atan2", "tanh")) + 4 * c["hypot_distinct"]
     steps = nsteps * nveh
-    return (basic + trans) / steps, basic / steps
+    return (basic + trans) / steps, basic / steps, (basic + dtrans) / steps
 
 
 def initial_records(O, dof, flat, n, ovr, seed=7):
@@ -215,18 +220,148 @@ def adjoint_cost(flat, n=65536, T=5000, K=100, mpc_iterations=20):
     torch.cuda.synchronize()
     mpc_s = time.perf_counter() - t0
     s.close()
-    return {"vehicles": n, "steps": T, "checkpoint_every": K, "tape_budget_bytes": whole,
-            "forward_ms": f_ms, "backward_ms": b_ms,
-            "ratio": (f_ms + b_ms) / f_ms, "plain_forward_ms": p_ms, "ratio_vs_plain_forward": (f_ms + b_ms) / p_ms,
+    fd_acc = central_fd_accuracy(flat, T, K)
+    whole_gb = whole / 1e9
+    return {"vehicles": n, "steps": T, "checkpoint_every": K,
+            "cost_ratio": (f_ms + b_ms) / p_ms,
+            "cost_ratio_definition": f"(adjoint forward + backward) / plain forward (integration kernel on the same "
+                                     f"controls), device times, tape budget {whole_gb:.1f} GB (the whole horizon: the "
+                                     f"forward tapes every step, the backward recomputes nothing)",
+            "tape_budget_bytes": whole, "forward_ms": f_ms, "backward_ms": b_ms, "plain_forward_ms": p_ms,
+            "ratio_vs_taping_forward": (f_ms + b_ms) / f_ms,
             "default_budget": {"tape_budget_bytes": default_budget, "forward_ms": fd_ms, "backward_ms": bd_ms,
-                               "ratio": (fd_ms + bd_ms) / fd_ms, "ratio_vs_plain_forward": (fd_ms + bd_ms) / p_ms},
+                               "cost_ratio": (fd_ms + bd_ms) / p_ms,
+                               "ratio_vs_taping_forward": (fd_ms + bd_ms) / fd_ms},
             "central_fd_cost_ratio": 2.0 * n_inputs * p_ms / (f_ms + b_ms),
-            "mpc": {"iterations": mpc_iterations, "seconds": mpc_s, "mean_loss_first": hist[0] if hist else None,
-                    "mean_loss_last": hist[-1] if hist else None},
-            "note": "ratio = (adjoint forward + backward) / adjoint forward device time, with a tape budget that "
-                    "holds the whole horizon (the forward tapes every step, the backward recomputes nothing); "
-                    "default_budget: the same at the library's default budget; central_fd_cost_ratio = "
-                    "cost of central differences (2 plain forwards per input) / adjoint cost"}
+            "central_fd_accuracy": fd_acc,
+            "mpc": {"iterations": mpc_iterations, "seconds": mpc_s, "seconds_per_iteration": mpc_s / max(1, mpc_iterations),
+                    "mean_loss_first": hist[0] if hist else None, "mean_loss_last": hist[-1] if hist else None},
+            "note": "central_fd_cost_ratio = cost of central differences (2 plain forwards per input, "
+                    f"{n_inputs} inputs per vehicle) / adjoint cost; central_fd_accuracy: the relative error central "
+                    "differences reach at their best step against the adjoint (the adjoint itself matches the "
+                    "dual-number oracle to ~1e-12, tests/test_gpu_adjoint.py)"}
+
+
+def central_fd_accuracy(flat, T=5000, K=100, nb=4, seed=3):
+    """Accuracy of central differences (J(x + h) - J(x - h)) / 2h of the adjoint workload's tracking loss,
+    the gradient whose cost central_fd_cost_ratio prices: for nb vehicles of the workload (moving start,
+    the MPC's feed-forward controls), 6 controls at random steps, the 7 per-vehicle parameters and 4
+    initial states (u, v, wz, front-left wheel speed), each differenced at relative steps 1e-2 ... 1e-7
+    on the GPU, against the adjoint's gradient.  Returns the best step's relative error per input
+    (median and worst over inputs): what central differences can reach at all."""
+    import torch
+
+    import lfdm_amd
+    from lfdm_amd import workloads as W
+    from lfdm_amd.mpc import arc_reference, initial_controls, moving_start
+
+    fields = lfdm_amd.record_fields(18)
+    rng = np.random.default_rng(seed)
+    ref, kappa = arc_reference(nb, T, device="cpu")
+    ref = ref.numpy()
+    c0 = initial_controls(flat, nb, T, kappa, device="cpu").numpy()
+    p0 = W.default_overrides(flat, nb)
+    i0 = moving_start(flat, nb).numpy()
+    inputs = [("c", (int(j), int(q))) for j, q in zip(rng.integers(0, T, 6), rng.integers(0, 2, 6))]
+    inputs += [("p", f) for f in range(7)] + [("s", fields.index(f)) for f in ("u", "v", "wz", "lf.omega")]
+    hs = (1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7)
+    per = 1 + 2 * len(inputs) * len(hs)
+    n = nb * per
+    C = np.repeat(c0, per, axis=2)
+    P = np.repeat(p0, per, axis=1)
+    I = np.repeat(i0, per, axis=0)
+    steps = {}
+    for b in range(nb):
+        for ii, (kind, d) in enumerate(inputs):
+            x0 = c0[d[0], d[1], b] if kind == "c" else p0[d, b] if kind == "p" else i0[b, d]
+            for hi, hr in enumerate(hs):
+                h = hr * max(abs(x0), 1e-2 if kind != "p" else abs(x0))
+                steps[(b, ii, hi)] = h
+                for o, sign in enumerate((1.0, -1.0)):
+                    col = b * per + 1 + 2 * (ii * len(hs) + hi) + o
+                    if kind == "c":
+                        C[d[0], d[1], col] += sign * h
+                    elif kind == "p":
+                        P[d, col] += sign * h
+                    else:
+                        I[col, d] += sign * h
+    s = lfdm_amd.SolverHalfImplicitGPU(18, n)
+    s.construct_flat(flat, n, None, lfdm_amd.TMeasy)
+    s.Initialize(s.zero_record(), n)
+    s.set_overrides(P)
+    s.set_states(I)
+    dc = torch.from_numpy(np.ascontiguousarray(C)).cuda()
+    xy = torch.empty((T, 2, n), dtype=torch.float64, device="cuda")
+    s.adjoint_forward(T, K, dc.data_ptr(), xy.data_ptr())
+    s.synchronize()
+    e = xy.cpu().numpy() - np.repeat(ref, per, axis=2)
+    J = 1e-3 * (e * e).sum(axis=(0, 1))
+    gxy = torch.from_numpy(np.ascontiguousarray(2e-3 * e)).cuda()
+    gc = torch.empty_like(dc)
+    gs = torch.empty((len(fields), n), dtype=torch.float64, device="cuda")
+    gp = torch.empty((7, n), dtype=torch.float64, device="cuda")
+    s.adjoint_backward(gxy.data_ptr(), gc.data_ptr(), gs.data_ptr(), gp.data_ptr())
+    s.synchronize()
+    s.close()
+    gc, gs, gp = gc.cpu().numpy(), gs.cpu().numpy(), gp.cpu().numpy()
+    best = []
+    for b in range(nb):
+        col0 = b * per
+        for ii, (kind, d) in enumerate(inputs):
+            adj = gc[d[0], d[1], col0] if kind == "c" else gp[d, col0] if kind == "p" else gs[d, col0]
+            errs = []
+            for hi in range(len(hs)):
+                col = col0 + 1 + 2 * (ii * len(hs) + hi)
+                fd = (J[col] - J[col + 1]) / (2 * steps[(b, ii, hi)])
+                errs.append(abs(fd - adj) / max(abs(adj), 1e-300))
+            best.append(min(errs))
+    best = np.array(best)
+    return {"inputs_per_vehicle": len(inputs), "vehicles": nb, "relative_steps": list(hs),
+            "best_step_relative_error": {"median": float(np.median(best)), "max": float(best.max())}}
+
+
+def n_sweep(H=10000, sizes=(1024, 2048, 4096, 8192, 16384, 32768, 65536, 131072, 262144, 393216, 524288)):
+    """The reference paper's N-sweep (paper/paper_results/bench.sh:32-42 with {11,18,24}dof_gpu_bench.cu):
+    HMMWV with TMeasy tires on the acc3 driver file, Construct + Initialize at rest, 10 s at dt = 1e-3,
+    output off, for each model and fleet size; one launch per run (device time from HIP events on the
+    solver's stream, wall time around the launch), after a short warm-up launch from the same records.
+    Beside it the reference's published A100 times of the same runs (paper/paper_results/data/LFDM),
+    for context."""
+    import lfdm_amd
+    from lfdm_amd import workloads as W
+
+    a100_ms = {  # paper/paper_results/data/LFDM/{11,18,24}dof.out, in the order of `sizes`
+        11: (301.165, 244.829, 269.287, 320.415, 457.012, 1293.57, 2234.26, 4089.2, 8278.1, 12275, 16362.9),
+        18: (485.186, 432.977, 460.563, 597.919, 982.565, 2821.58, 4877.27, 9150.85, 18626.2, 27482.3, 36857.3),
+        24: (843.603, 794.223, 840.184, 1169.15, 2809.72, 6268.76, 13076.7, 24967.7, 49432.2, 73988, 98794.6)}
+    out = {}
+    acc3 = os.path.join(W.DATA, "input", "acc3.txt")
+    for dof in (18, 11, 24):
+        d = os.path.join(W.DATA, f"{dof}dof", "HMMWV")
+        files = [f"{d}/vehicle.json", f"{d}/tmeasy.json"] + ([f"{d}/suspension.json"] if dof == 24 else [])
+        rows = []
+        for N, ref_ms in zip(sizes, a100_ms[dof]):
+            s = lfdm_amd.SolverHalfImplicitGPU(dof, N)
+            s.Construct(*files, N, acc3)
+            s.Initialize(s.zero_record(), N)
+            s.SetThreadsPerBlock(64)
+            init = s.get_states()
+            s.advance(200)  # warm-up: builds the device image
+            s.set_states(init)
+            s.set_time(0.0)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.advance(H)
+            s.synchronize()
+            wall = time.perf_counter() - t0
+            dev_ms = s.last_launch_ms()
+            s.close()
+            rows.append({"vehicles": N, "device_ms": dev_ms, "wall_ms": 1e3 * wall,
+                         "vehicle_steps_per_s": N * H / wall, "a100_ms": ref_ms,
+                         "a100_vehicle_steps_per_s": N * H / (ref_ms * 1e-3)})
+        out[f"{dof}dof"] = rows
+    return {"workload": "HMMWV TMeasy, acc3 driver file, start at rest, 10 s at dt=1e-3, output off "
+                        "(the reference paper's GPU benchmark)", "sizes": list(sizes), "models": out}
 
 
 def hmmwv_params(dof):
@@ -412,6 +547,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the batching-form (mixed fleet) measurement")
     ap.add_argument("--no-other-models", action="store_true", help="skip the 11- and 24-DOF lines of the default run")
     ap.add_argument("--mixed-only", action="store_true", help="only the batching-form measurement")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the reference paper's N-sweep (acc3, 1,024 .. "
+                                                            "524,288 vehicles, all three models)")
+    ap.add_argument("--sweep-only", action="store_true", help="only the N-sweep")
     ap.add_argument("--mixed-vehicles", type=int, default=2_000_000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (reads 1 GiB, writes 1 GiB), the "
@@ -436,6 +574,10 @@ def main():
         print(json.dumps(mixed_fleet(args.mixed_vehicles)))
         return
 
+    if args.sweep_only:
+        print(json.dumps(n_sweep(args.horizon_steps)))
+        return
+
     if args.adjoint_only:
         print(json.dumps(adjoint_cost(hmmwv_params(18), mpc_iterations=args.mpc_iterations)))
         return
@@ -444,13 +586,18 @@ def main():
     flat = hmmwv_params(dof)
     value, wall, kernel_ms = knot_fleet_rate(dof, flat, n, H, args.steps, args.warmup, args.tpb)
     ms_per_step = 1e3 * wall / args.steps
-    fl_total, fl_basic = flops_per_step(dof, flat)
+    fl_total, fl_basic, fl_exec = flops_per_step(dof, flat)
     achieved_tf = fl_total * n * H / (kernel_ms * 1e-3) / 1e12
     tps = measured_traffic_per_step() if dof == 18 else None
-    roofline = {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+    roofline = {"bound": "fp64_valu", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None if tps is None else tps * n * H,
+                "bound_note": "FP64 vector (VALU) issue: the step has no dense contraction and uses no MFMA; peak = "
+                              "MI355X's published FP64 figure",
                 "kernel": f"integrate<M{dof}, TMeasy, knots>", "kernel_ms": kernel_ms,
                 "flop_per_vehicle_step": fl_total, "basic_flop_per_vehicle_step": fl_basic,
+                "flop_per_vehicle_step_distinct_libm": fl_exec,
+                "frac_basic_ops": fl_basic * n * H / (kernel_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                "frac_distinct_libm": fl_exec * n * H / (kernel_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
                 "traffic_source": "profiles/integrate_18dof_knot_fleet_counters.md (bytes per vehicle-step x "
                                   "vehicle-steps of one launch)" if tps is not None else None,
                 "hbm_gbs": None if tps is None else tps * n * H / (kernel_ms * 1e-3) / 1e9,
@@ -484,16 +631,22 @@ def main():
         other = {}
         for d_, n_ in ((24, 500_000), (11, 1_000_000)):
             v_, _, k_ = knot_fleet_rate(d_, hmmwv_params(d_), n_, H, 2, 1, args.tpb)
-            f_, _ = flops_per_step(d_, hmmwv_params(d_))
+            f_, fb_, fe_ = flops_per_step(d_, hmmwv_params(d_))
+            tf_ = lambda f: f * n_ * H / (k_ * 1e-3) / 1e12 / FP64_PEAK_TFLOPS  # noqa: E731
             other[f"{d_}dof"] = {"value": v_, "unit": "vehicle-steps/s", "vehicles": n_, "horizon_steps": H,
-                                 "steps": 2, "kernel_ms": k_,
-                                 "fp64_frac": f_ * n_ * H / (k_ * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+                                 "steps": 2, "kernel_ms": k_, "flop_per_vehicle_step": f_,
+                                 "flop_per_vehicle_step_distinct_libm": fe_, "basic_flop_per_vehicle_step": fb_,
+                                 "fp64_frac": tf_(f_), "fp64_frac_distinct_libm": tf_(fe_),
+                                 "fp64_frac_basic_ops": tf_(fb_)}
     adj = None
     if dof == 18 and not args.no_adjoint:
         adj = adjoint_cost(flat, mpc_iterations=args.mpc_iterations)
     mixed = None
     if dof == 18 and not args.no_mixed:
         mixed = mixed_fleet(args.mixed_vehicles)
+    sweep = None
+    if dof == 18 and not args.no_sweep:
+        sweep = n_sweep(H)
     line = {"metric": metric, "value": value,
             "unit": "vehicle-steps/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -504,8 +657,9 @@ def main():
                                    (", start at U(0,20) m/s" if dof == 24 else ""),
                        "vehicles": n, "horizon_steps": H, "dt": 1e-3, "threads_per_block": args.tpb},
             "roofline": roofline, "cpu_baseline": cpu, "adjoint": adj,
-            "adjoint_forward_cost_ratio": None if adj is None else adj["ratio"], "mixed_fleet": mixed,
-            "other_models": other}
+            "adjoint_forward_cost_ratio": None if adj is None else adj["cost_ratio"],
+            "adjoint_cost_ratio_definition": None if adj is None else adj["cost_ratio_definition"],
+            "mixed_fleet": mixed, "other_models": other, "n_sweep": sweep}
     print(json.dumps(line))
 
 

@@ -121,8 +121,14 @@ int lfdm_get_states(lfdm_solver* h, double* records);
 int lfdm_set_states(lfdm_solver* h, const double* records);
 
 /* Per-vehicle parameter values ovr[7][total]: m, jz, jx, dfy0Pn, dfy0P2n, torque_scale (multiplies
- * the engine torque-map ordinates), maxBrakeTorque.  Defaults come from each vehicle's group. */
+ * the engine torque-map ordinates), maxBrakeTorque.  Defaults come from each vehicle's group.  A pending
+ * adjoint (lfdm_adjoint_forward without its backward yet) reads them again, so changing them cancels it. */
 int lfdm_set_overrides(lfdm_solver* h, const double* ovr);
+/* Device-pointer forms of lfdm_set_overrides and lfdm_set_states (device_ovr[7][total],
+ * device_records[total][record_len]): copied on the handle's stream without synchronizing and without
+ * passing through the host (e.g. from torch tensors every optimization step); values are not checked. */
+int lfdm_set_overrides_device(lfdm_solver* h, const double* device_ovr);
+int lfdm_set_states_device(lfdm_solver* h, const double* device_records);
 
 /* Control sources for Solve (the default is each group's driver table):
  *  knots: steering linear on a grid of ds seconds (ns knots), throttle and braking piecewise
@@ -185,6 +191,10 @@ int lfdm_adjoint_backward(lfdm_solver* h, const double* device_xy_bar, double* d
  * horizon (steps x 38 x total x 8 bytes) makes the forward tape every step and the backward recompute
  * nothing.  Results do not depend on it. */
 int lfdm_adjoint_set_tape_budget(lfdm_solver* h, unsigned long long bytes);
+/* Counter of the handle's adjoint state: it changes with every lfdm_adjoint_forward and whenever a
+ * pending adjoint is cancelled (new per-vehicle parameters, a new slot layout).  A caller holding a
+ * forward's value can tell whether its backward is still the one that forward set up. */
+int lfdm_adjoint_generation(lfdm_solver* h, unsigned long long* generation);
 /* Exact Jacobian of one step at every vehicle's current record, for linearizing the model (e.g. MPC).
  * Replaces d18SolverHalfImplicit::IntegrateStepWithJacobian + GetJacobianState / GetJacobianControls
  * (18dof/dof18_halfImplicit.h:79-86, 159; dof18_halfImplicit.cpp:356-463), which differences the

@@ -1,7 +1,7 @@
 #include "json.h"
 
 #include <cctype>
-#include <cstdlib>
+#include <charconv>
 #include <fstream>
 #include <sstream>
 
@@ -16,10 +16,14 @@ const JValue* JValue::find(const std::string& key) const {
 
 namespace {
 
+// Nesting deeper than this is refused (the reference's parameter files nest three levels).
+constexpr int kMaxDepth = 64;
+
 struct Parser {
     const std::string& s;
     size_t i = 0;
     std::string err;
+    int depth = 0;
 
     void ws() {
         while (i < s.size() && std::isspace((unsigned char)s[i])) ++i;
@@ -54,6 +58,12 @@ struct Parser {
         return true;
     }
     bool value(JValue& v) {
+        if (++depth > kMaxDepth) return fail("nesting too deep");
+        const bool ok = value_at(v);
+        --depth;
+        return ok;
+    }
+    bool value_at(JValue& v) {
         ws();
         if (i >= s.size()) return fail("unexpected end");
         char c = s[i];
@@ -99,12 +109,13 @@ struct Parser {
         if (c == 'f') { v.kind = JValue::Bool; v.b = false; return lit("false"); }
         if (c == 'n') { v.kind = JValue::Null; return lit("null"); }
         if (c == '-' || (c >= '0' && c <= '9')) {
-            const char* p = s.c_str() + i;
-            char* e = nullptr;
+            // std::from_chars: correctly rounded and independent of the process's C locale (strtod
+            // would read "0.5" as 0 under a comma-decimal LC_NUMERIC)
+            const char* p = s.data() + i;
             v.kind = JValue::Number;
-            v.num = std::strtod(p, &e);
-            if (e == p) return fail("bad number");
-            i += (size_t)(e - p);
+            const auto r = std::from_chars(p, s.data() + s.size(), v.num);
+            if (r.ec != std::errc() || r.ptr == p) return fail("bad number");
+            i += (size_t)(r.ptr - p);
             return true;
         }
         return fail("unexpected character");

@@ -107,6 +107,9 @@ int set_last_error(int code, const char* msg);
 // Gathers full records of `n` slots from SoA state into out[n][rec].
 hipError_t launch_gather(const double* state, int S, int rec, const int* slots, int n, double* out,
                          hipStream_t stream);
+// Scatters field-major rows src[F][n] (vehicle v in column v) into dst[F][S] at user_slot[v].
+hipError_t launch_scatter_fields(double* dst, int S, int F, const int* user_slot, int n, const double* src,
+                                 hipStream_t stream);
 // Scatters records in[n][rec] into the SoA state at `slots`.
 hipError_t launch_scatter(double* state, int S, int rec, const int* slots, int n, const double* in,
                           hipStream_t stream);

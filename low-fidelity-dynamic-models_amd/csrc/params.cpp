@@ -2,6 +2,7 @@
 
 #include <cmath>
 #include <fstream>
+#include <locale>
 #include <sstream>
 
 #include "json.h"
@@ -485,6 +486,7 @@ bool load_driver_file(const std::string& path, std::vector<DrvRow>& rows, std::s
     std::string line;
     while (std::getline(f, line)) {
         std::istringstream iss(line);
+        iss.imbue(std::locale::classic());  // '.' decimals whatever the process's locale
         DrvRow r;
         iss >> r.t >> r.steering >> r.throttle >> r.braking;
         if (iss.fail()) break;

@@ -23,7 +23,25 @@ __global__ void scatter_records(double* __restrict__ state, int S, int rec, cons
     state[(size_t)f * S + slots[v]] = in[i];
 }
 
+__global__ void scatter_field_rows(double* __restrict__ dst, int S, int F, const int* __restrict__ user_slot, int n,
+                                   const double* __restrict__ src) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)n * F) return;
+    const int f = (int)(i / n), v = (int)(i % n);
+    dst[(size_t)f * S + user_slot[v]] = src[i];
+}
+
 }  // namespace
+
+hipError_t launch_scatter_fields(double* dst, int S, int F, const int* user_slot, int n, const double* src,
+                                 hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const long long total = (long long)n * F;
+    const int block = 256;
+    const int grid = (int)((total + block - 1) / block);
+    hipLaunchKernelGGL(scatter_field_rows, grid, block, 0, stream, dst, S, F, user_slot, n, src);
+    return hipGetLastError();
+}
 
 hipError_t launch_gather(const double* state, int S, int rec, const int* slots, int n, double* out,
                          hipStream_t stream) {

@@ -113,6 +113,7 @@ struct lfdm_solver {
     bool steps_dirty = false;
     std::vector<double> ovr_host;  // [7][total]
     bool ovr_dirty = false;
+    bool ovr_device_newer = false;  // lfdm_set_overrides_device wrote the device copy after ovr_host
     double t_now = 0.0;
     // per-vehicle horizons (steps from t = 0; empty: none) and the slot layout
     std::vector<int> horizon_host;
@@ -157,6 +158,7 @@ struct lfdm_solver {
     double* d_carry = nullptr;
     size_t carry_cap = 0;
     int adj_nsteps = 0, adj_K = 0;
+    unsigned long long adj_gen = 0;  // adjoint forwards run on this handle (and pending ones invalidated)
     int adj_G = 1;                 // segments per backward group, fixed by the forward
     bool tape_holds_last = false;  // the tape holds the last group, written by the forward
     size_t adj_tape_budget = size_t(16) << 30;  // bytes the backward's tape may take
@@ -194,6 +196,15 @@ namespace {
 int check(lfdm_solver* h) {
     if (!h) return fail(LFDM_E_ARG, "null solver handle");
     return LFDM_OK;
+}
+
+// A pending adjoint (forward run, backward not yet) reads the per-vehicle parameters again: changing
+// them makes it unusable.
+void invalidate_adjoint(lfdm_solver* h) {
+    if (h->adj_nsteps > 0) {
+        h->adj_nsteps = 0;
+        ++h->adj_gen;
+    }
 }
 
 int ensure_buffer(double*& p, size_t& cap, size_t n) {
@@ -318,6 +329,16 @@ int drop_device_image(lfdm_solver* h) {
     if (int rc = lfdm_get_states(h, cur.data())) return rc;
     h->init.swap(cur);
     h->n_states = h->total;
+    if (h->ovr_device_newer) {  // the device copy is the current one: keep it for the rebuild
+        std::vector<double> ovr((size_t)kOverrides * h->S);
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemcpy(ovr.data(), h->d_ovr, ovr.size() * sizeof(double), hipMemcpyDeviceToHost));
+        h->ovr_host.assign((size_t)kOverrides * h->total, 0.0);
+        for (unsigned u = 0; u < h->total; ++u)
+            for (int f = 0; f < kOverrides; ++f)
+                h->ovr_host[(size_t)f * h->total + u] = ovr[(size_t)f * h->S + h->user_slot[u]];
+        h->ovr_device_newer = false;
+    }
     dfree(h->d_sets);
     dfree(h->d_wave_set);
     dfree(h->d_slot_user);
@@ -328,7 +349,7 @@ int drop_device_image(lfdm_solver* h) {
     dfree(h->d_horizon);
     dfree(h->d_wave_hmax);
     h->built = false;
-    h->adj_nsteps = 0;  // a pending adjoint refers to the old slots
+    invalidate_adjoint(h);  // a pending adjoint refers to the old slots
     h->tape_holds_last = false;
     return LFDM_OK;
 }
@@ -1015,6 +1036,34 @@ int lfdm_set_overrides(lfdm_solver* h, const double* ovr) {
         if (!std::isfinite(ovr[i])) return fail(LFDM_E_ARG, "per-vehicle parameters must be finite");
     h->ovr_host.assign(ovr, ovr + (size_t)kOverrides * h->total);
     if (h->built) h->ovr_dirty = true;
+    h->ovr_device_newer = false;
+    invalidate_adjoint(h);
+    return LFDM_OK;
+}
+
+int lfdm_set_overrides_device(lfdm_solver* h, const double* device_ovr) {
+    if (int rc = check(h)) return rc;
+    if (!device_ovr) return fail(LFDM_E_ARG, "null overrides");
+    if (int rc = ensure_device(h)) return rc;
+    if (int rc = sync_inputs(h)) return rc;
+    HIPCHK(launch_scatter_fields(h->d_ovr, h->S, kOverrides, h->d_user_slot, (int)h->total, device_ovr, h->stream));
+    h->ovr_device_newer = true;
+    invalidate_adjoint(h);
+    return LFDM_OK;
+}
+
+int lfdm_set_states_device(lfdm_solver* h, const double* device_records) {
+    if (int rc = check(h)) return rc;
+    if (!device_records) return fail(LFDM_E_ARG, "null input");
+    if (int rc = ensure_device(h)) return rc;
+    HIPCHK(launch_scatter(h->d_state, h->S, h->rec, h->d_user_slot, (int)h->total, device_records, h->stream));
+    return LFDM_OK;
+}
+
+int lfdm_adjoint_generation(lfdm_solver* h, unsigned long long* gen) {
+    if (int rc = check(h)) return rc;
+    if (!gen) return fail(LFDM_E_ARG, "null output");
+    *gen = h->adj_gen;
     return LFDM_OK;
 }
 
@@ -1234,6 +1283,7 @@ int lfdm_adjoint_forward(lfdm_solver* h, int nsteps, int checkpoint_every, const
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     h->timed = true;
     h->tape_holds_last = true;
+    ++h->adj_gen;
     return LFDM_OK;
 }
 
@@ -1264,7 +1314,9 @@ int lfdm_adjoint_backward(lfdm_solver* h, const double* dxy_bar, double* dctrl_b
                           double* dparam_bar) {
     if (int rc = check(h)) return rc;
     if (!dxy_bar || !dctrl_bar || !dstate_bar || !dparam_bar) return fail(LFDM_E_ARG, "null adjoint buffer");
-    if (h->adj_nsteps <= 0) return fail(LFDM_E_STATE, "adjoint backward before adjoint forward");
+    if (h->adj_nsteps <= 0)
+        return fail(LFDM_E_STATE, "no adjoint forward pending: none ran, or the per-vehicle parameters or the "
+                                  "slot layout changed since");
     const int nseg = (h->adj_nsteps + h->adj_K - 1) / h->adj_K;
     const int G = h->adj_G;
     if (int rc = ensure_buffer(h->d_carry, h->carry_cap, (size_t)kAdjCarry * (size_t)h->S)) return rc;

@@ -87,6 +87,9 @@ SIGNATURES = {
     "lfdm_get_states": [_p, _p],
     "lfdm_set_states": [_p, _p],
     "lfdm_set_overrides": [_p, _p],
+    "lfdm_set_overrides_device": [_p, _p],
+    "lfdm_set_states_device": [_p, _p],
+    "lfdm_adjoint_generation": [_p, ctypes.POINTER(ctypes.c_ulonglong)],
     "lfdm_set_controls_knots": [_p, _i, _d, _i, _d, _p],
     "lfdm_set_controls_steps": [_p, _i, _p],
     "lfdm_set_controls_steps_device": [_p, _i, _p],

@@ -62,8 +62,12 @@ def initial_controls(flat, n, nsteps, kappa, device="cuda"):
 
 
 def run_mpc(solver, flat, init, params, ref, ctrl0, iterations=20, lr=0.05, checkpoint_every=100):
-    """Adam on the controls; returns (controls, per-iteration mean loss)."""
+    """Adam on the controls; returns (controls, per-iteration mean loss).  params and init move to the
+    controls' device once; the iterations queue work without waiting for the GPU (the losses are read
+    back after the last one)."""
     ctrl = ctrl0.clone().requires_grad_(True)
+    params = params.to(ctrl.device, torch.float64)
+    init = init.to(ctrl.device, torch.float64)
     opt = torch.optim.Adam([ctrl], lr=lr)
     history = []
     for _ in range(iterations):
@@ -75,5 +79,5 @@ def run_mpc(solver, flat, init, params, ref, ctrl0, iterations=20, lr=0.05, chec
         with torch.no_grad():
             ctrl[:, 0].clamp_(-1.0, 1.0)
             ctrl[:, 1:].clamp_(0.0, 1.0)
-        history.append(loss.item() / ctrl.shape[2])
-    return ctrl.detach(), history
+        history.append(loss.detach())
+    return ctrl.detach(), (torch.stack(history) / ctrl.shape[2]).tolist()

@@ -217,6 +217,14 @@ class SolverHalfImplicitGPU:
         assert ovr.shape == (len(OVERRIDE_FIELDS), self.total)
         check(lib.lfdm_set_overrides(self._h, ovr.ctypes.data))
 
+    def set_overrides_device(self, device_ptr):
+        """Per-vehicle parameters from device memory, (7, total) float64 (e.g. a torch tensor's data_ptr())."""
+        check(lib.lfdm_set_overrides_device(self._h, ctypes.c_void_p(device_ptr)))
+
+    def set_states_device(self, device_ptr):
+        """Records from device memory, (total, record_len) float64."""
+        check(lib.lfdm_set_states_device(self._h, ctypes.c_void_p(device_ptr)))
+
     def set_controls_knots(self, ns, ds, np_, dp, knots):
         knots = _f64(knots)
         assert knots.shape == (ns + 2 * np_, self.total)
@@ -281,6 +289,11 @@ class SolverHalfImplicitGPU:
 
     def adjoint_set_tape_budget(self, nbytes):
         check(lib.lfdm_adjoint_set_tape_budget(self._h, int(nbytes)))
+
+    def adjoint_generation(self):
+        v = ctypes.c_ulonglong()
+        check(lib.lfdm_adjoint_generation(self._h, ctypes.byref(v)))
+        return v.value
 
     def step_jacobian(self, ctrl_ptr, jac_ptr):
         """Exact Jacobian of one step at every vehicle's current record (device pointers: controls

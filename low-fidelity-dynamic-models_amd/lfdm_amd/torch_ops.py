@@ -5,7 +5,13 @@ from given initial records with per-step controls and per-vehicle parameters and
 trajectory; its backward pass is the hand-written reverse-mode adjoint of the half-implicit step
 (lfdm_adjoint_forward / lfdm_adjoint_backward, include/lfdm_amd.h), with checkpoints every
 ``checkpoint_every`` steps.  Gradients flow to controls, params and init.  Tensors are float64;
-controls and the returned trajectory live on the GPU, params and init may live anywhere.
+controls and the returned trajectory live on the GPU, params and init may live anywhere (they are
+handed to the solver from device memory, so on the GPU nothing passes through the host).
+
+The handle keeps one pending adjoint: a second rollout on the same solver (or new per-vehicle
+parameters) before the first one's backward replaces it, and the first backward then raises instead of
+returning gradients of the wrong trajectory.  Use one solver per rollout that is differentiated
+together.
 """
 import numpy as np
 import torch
@@ -22,11 +28,17 @@ class HalfImplicitRollout(torch.autograd.Function):
         if controls.dtype != torch.float64 or not controls.is_cuda:
             raise ValueError("controls must be a float64 CUDA tensor")
         solver.set_stream(torch.cuda.current_stream(controls.device).cuda_stream)
-        solver.set_overrides(params.detach().to("cpu", torch.float64).numpy())
-        solver.set_states(init.detach().to("cpu", torch.float64).numpy())
+        p_dev = params.detach().to(controls.device, torch.float64).contiguous()
+        i_dev = init.detach().to(controls.device, torch.float64).contiguous()
+        if tuple(p_dev.shape) != (len(OVERRIDE_FIELDS), n) or tuple(i_dev.shape) != (n, record_len(18, TMeasy)):
+            raise ValueError("params must be (7, n) and init (n, record_len)")
+        solver.set_overrides_device(p_dev.data_ptr())
+        solver.set_states_device(i_dev.data_ptr())
         ctrl = controls.detach().contiguous()
         xy = torch.empty((T, 2, n), dtype=torch.float64, device=controls.device)
         solver.adjoint_forward(T, int(checkpoint_every), ctrl.data_ptr(), xy.data_ptr())
+        ctx.generation = solver.adjoint_generation()
+        ctx.keep = (p_dev, i_dev)  # read on the stream after this call returns
         ctx.solver = solver
         ctx.params_device = params.device
         ctx.init_device = init.device
@@ -37,6 +49,10 @@ class HalfImplicitRollout(torch.autograd.Function):
     def backward(ctx, grad_xy):
         (ctrl,) = ctx.saved_tensors
         s = ctx.solver
+        if s.adjoint_generation() != ctx.generation:
+            raise RuntimeError("the solver ran another adjoint forward (or got new per-vehicle parameters) after "
+                               "this rollout, so its backward state is gone; use one solver per rollout that is "
+                               "differentiated together")
         g = grad_xy.contiguous().to(torch.float64)
         gctrl = torch.empty_like(ctrl)
         gstate = torch.empty((record_len(18, TMeasy), s.total), dtype=torch.float64, device=ctrl.device)

@@ -18,6 +18,8 @@
 
 namespace orc {
 thread_local OpCount g_ops;
+thread_local SeenCall g_seen[512];
+thread_local int g_nseen = 0;
 }
 
 using namespace orc;
@@ -323,6 +325,7 @@ int orc18_count_ops(int nr, const double* flat, const double* init, double t0, d
     for (int k = 0; k < nsteps; ++k) {
         double st, th, br;
         cs.at(0, k, t, st, th, br);
+        new_step();
         d18::step(s, CD(st), CD(th), CD(br), dt);
         t = t + dt;
     }
@@ -349,6 +352,7 @@ int orc_count_ops(int dof, int nr, const double* flat, const double* init, doubl
         for (int k = 0; k < nsteps; ++k) {
             double st, th, br;
             cs.at(0, k, t, st, th, br);
+            new_step();
             M::step(s, CD(st), CD(th), CD(br), dt);
             t = t + dt;
         }
@@ -357,10 +361,36 @@ int orc_count_ops(int dof, int nr, const double* flat, const double* init, doubl
     else if (dof == 18) run(M18{});
     else if (dof == 24) run(M24{});
     else return -1;
-    const long long c[] = {g_ops.add, g_ops.mul,  g_ops.div,   g_ops.sqrt,  g_ops.sin,
-                           g_ops.cos, g_ops.tan,  g_ops.atan2, g_ops.hypot, g_ops.tanh};
+    const long long c[] = {g_ops.add,  g_ops.mul,  g_ops.div,   g_ops.sqrt,   g_ops.sin,  g_ops.cos,
+                           g_ops.tan,  g_ops.atan2, g_ops.hypot, g_ops.tanh,  g_ops.dsin, g_ops.dcos,
+                           g_ops.dtan, g_ops.datan2, g_ops.dhypot, g_ops.dtanh};
     std::memcpy(counts, c, sizeof(c));
     return 0;
+}
+
+// Tallies of single reference functions, for checks against counts made by hand from the reference
+// source: computeVehRHS (dof18.cpp:1019-1078) at a record with given tire forces, and tmxy_combined
+// (dof18.cpp:247-288) at one operating point.  counts as orc_count_ops.
+void orc_count_veh_rhs18(const double* flat, const double* rec, const double* fx, const double* fy,
+                         long long* counts) {
+    d18::Vehicle<CD> s;
+    d18::read_params(flat, false, s.vp, s.tp);
+    d18::from_record(s, rec);
+    CD fxc[4], fyc[4];
+    for (int i = 0; i < 4; ++i) {
+        fxc[i] = CD(fx[i]);
+        fyc[i] = CD(fy[i]);
+    }
+    g_ops = OpCount{};
+    d18::veh_rhs(s.vs, s.vp, fxc, fyc);
+    counts[0] = g_ops.add; counts[1] = g_ops.mul; counts[2] = g_ops.div; counts[3] = g_ops.sqrt;
+}
+
+void orc_count_tmxy(double s, double df0, double sm, double fm, double ss, double fs, long long* counts) {
+    CD f, fos;
+    g_ops = OpCount{};
+    d18::tmxy_combined(f, fos, CD(s), CD(df0), CD(sm), CD(fm), CD(ss), CD(fs));
+    counts[0] = g_ops.add; counts[1] = g_ops.mul; counts[2] = g_ops.div; counts[3] = g_ops.sqrt;
 }
 
 // ---- building blocks, exposed for closed-form and invariant checks ----

@@ -49,6 +49,10 @@ def lib():
         L.orc18_count_ops.restype = i
         L.orc_count_ops.argtypes = [i, i, p, p, d, d, i, i, p, p, p, p]
         L.orc_count_ops.restype = i
+        L.orc_count_veh_rhs18.argtypes = [p, p, p, p, p]
+        L.orc_count_veh_rhs18.restype = None
+        L.orc_count_tmxy.argtypes = [d, d, d, d, d, d, p]
+        L.orc_count_tmxy.restype = None
         L.orc_tmxy.argtypes = [d, d, d, d, d, d, p, p]
         L.orc_tmxy.restype = None
         for fn in ("orc_interp_l", "orc_interp_q"):
@@ -167,16 +171,37 @@ def count_ops18(flat, controls, nsteps, tire_nr=False, dt=1e-3):
     return dict(zip(OP_NAMES, c.tolist()))
 
 
+# distinct-argument tallies: a libm call repeating one of the same step with the same argument is
+# counted in the plain tally only
+DISTINCT_NAMES = ("sin_distinct", "cos_distinct", "tan_distinct", "atan2_distinct", "hypot_distinct", "tanh_distinct")
+
+
 def count_ops(dof, flat, controls, nsteps, tire_nr=False, init=None, dt=1e-3):
-    """Operation tallies over nsteps of one vehicle of the 11-, 18- or 24-DOF model."""
+    """Operation tallies over nsteps of one vehicle of the 11-, 18- or 24-DOF model (OP_NAMES, then
+    DISTINCT_NAMES)."""
     L = lib()
     flat = np.ascontiguousarray(flat, dtype=np.float64)
     init = None if init is None else np.ascontiguousarray(init, dtype=np.float64)
     mode, ctrl, im, dm = _ctrl(controls)
-    c = np.zeros(len(OP_NAMES), dtype=np.int64)
+    c = np.zeros(len(OP_NAMES) + len(DISTINCT_NAMES), dtype=np.int64)
     assert L.orc_count_ops(dof, int(tire_nr), _ptr(flat), _ptr(init), 0.0, dt, nsteps, mode, _ptr(ctrl), _ptr(im),
                            _ptr(dm), _ptr(c)) == 0
-    return dict(zip(OP_NAMES, c.tolist()))
+    return dict(zip(OP_NAMES + DISTINCT_NAMES, c.tolist()))
+
+
+def count_veh_rhs18(flat, rec, fx, fy):
+    """(add, mul, div, sqrt) of one computeVehRHS evaluation."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (flat, rec, fx, fy)]
+    c = np.zeros(4, dtype=np.int64)
+    lib().orc_count_veh_rhs18(*[x.ctypes.data for x in a], c.ctypes.data)
+    return tuple(c.tolist())
+
+
+def count_tmxy(s, df0, sm, fm, ss, fs):
+    """(add, mul, div, sqrt) of one tmxy_combined evaluation."""
+    c = np.zeros(4, dtype=np.int64)
+    lib().orc_count_tmxy(s, df0, sm, fm, ss, fs, c.ctypes.data)
+    return tuple(c.tolist())
 
 
 def tmxy(s, df0, sm, fm, ss, fs):

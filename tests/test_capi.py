@@ -186,3 +186,40 @@ def test_noalloc_refuses_fleets_beyond_its_array():
     rc = lib.lfdm_integrate_noalloc(18, flat.ctypes.data, flat.size, lfdm_amd.TMeasy, 0, 8, None, 8, 10, 1e-3, 0, 8,
                                     ctypes.c_void_p(0))
     assert rc == -1
+
+
+def test_json_nesting_is_capped(tmp_path):
+    """A parameter file nested beyond 64 levels is refused with a parse error, not a stack overflow."""
+    deep = tmp_path / "deep.json"
+    deep.write_text("[" * 100000 + "]" * 100000)
+    veh = f"{DATA}/18dof/HMMWV/vehicle.json"
+    with pytest.raises(LfdmError) as e:
+        lfdm_amd.params_from_json(18, veh, str(deep))
+    assert e.value.name == "LFDM_E_PARSE" and "nesting too deep" in str(e.value)
+
+
+def test_parsing_ignores_the_process_locale():
+    """Numbers parse the same under a comma-decimal C locale (run where such a locale is installed)."""
+    import locale
+    import subprocess
+    import sys
+
+    names = [n for n in ("de_DE.UTF-8", "de_DE.utf8", "fr_FR.UTF-8", "fr_FR.utf8", "de_DE", "fr_FR")]
+    usable = None
+    for n in names:
+        try:
+            old = locale.setlocale(locale.LC_NUMERIC)
+            locale.setlocale(locale.LC_NUMERIC, n)
+            locale.setlocale(locale.LC_NUMERIC, old)
+            usable = n
+            break
+        except locale.Error:
+            continue
+    if usable is None:
+        pytest.skip("no comma-decimal locale installed")
+    veh, tire = f"{DATA}/18dof/HMMWV/vehicle.json", f"{DATA}/18dof/HMMWV/tmeasy.json"
+    code = ("import locale, sys; sys.path.insert(0, %r); import lfdm_amd; locale.setlocale(locale.LC_ALL, %r); "
+            "print(lfdm_amd.params_from_json(18, %r, %r).tobytes().hex())"
+            % (os.path.join(ROOT, "low-fidelity-dynamic-models_amd"), usable, veh, tire))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True).stdout.strip()
+    assert bytes.fromhex(out) == lfdm_amd.params_from_json(18, veh, tire).tobytes()

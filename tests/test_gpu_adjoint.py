@@ -284,6 +284,46 @@ def test_torch_rollout_backward_and_mpc_step():
     s.close()
 
 
+def test_two_rollouts_on_one_solver_refuse_a_stale_backward():
+    """The handle keeps one pending adjoint: two rollouts on one solver summed into one loss make the
+    first rollout's backward raise (its checkpoints were overwritten) instead of returning gradients of
+    the wrong trajectory; so do new per-vehicle parameters between a forward and its backward.  With one
+    solver per rollout the summed loss's gradients are the sum of the separate ones, bit for bit."""
+    from lfdm_amd.mpc import arc_reference, initial_controls, moving_start, tracking_loss
+    from lfdm_amd.torch_ops import default_params, make_solver, rollout
+
+    flat = hmmwv()
+    n, T = 128, 300
+    ref, kappa = arc_reference(n, T, seed=6)
+    init = moving_start(flat, n)
+    params = default_params(flat, n)
+    c0 = initial_controls(flat, n, T, kappa)
+    s = make_solver(flat, n)
+    ctrl = c0.clone().requires_grad_(True)
+    loss = tracking_loss(rollout(s, ctrl, params, init), ref).sum() + \
+        tracking_loss(rollout(s, 0.9 * ctrl, params, init), ref).sum()
+    with pytest.raises(RuntimeError, match="another adjoint forward"):
+        loss.backward()
+    ctrl.grad = None
+    loss = tracking_loss(rollout(s, ctrl, params, init), ref).sum()
+    s.set_overrides(params.numpy())
+    with pytest.raises(RuntimeError, match="another adjoint forward"):
+        loss.backward()
+    s2 = make_solver(flat, n)
+    ctrl.grad = None
+    (tracking_loss(rollout(s, ctrl, params, init), ref).sum() +
+     tracking_loss(rollout(s2, 0.9 * ctrl, params, init), ref).sum()).backward()
+    both = ctrl.grad.clone()
+    ctrl.grad = None
+    tracking_loss(rollout(s, ctrl, params, init), ref).sum().backward()
+    g1 = ctrl.grad.clone()
+    ctrl.grad = None
+    tracking_loss(rollout(s2, 0.9 * ctrl, params, init), ref).sum().backward()
+    assert torch.equal(both, g1 + ctrl.grad)
+    s.close()
+    s2.close()
+
+
 def test_default_stream_orders_with_torch():
     """A handle handed torch's default stream (NULL) before its first launch launches on it: torch work
     queued after the adjoint forward sees the finished trajectory without an explicit synchronize,

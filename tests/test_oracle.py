@@ -14,6 +14,7 @@ import pytest
 import lfdm_oracle as O
 import refbind
 from cases18 import CASES, KAT_X, inputs, normwise_rel, paths
+from conftest import DATA
 
 # record fields of a TMeasy vehicle that the reference leaves uninitialized when the vehicle has no
 # torque converter (VehicleState has no initializer for _dOmega_crank, dof18.h:193-210)
@@ -200,3 +201,44 @@ def test_reference_model_is_ulp_sensitive_for_some_vehicles(golden18):
     spread = ulp_spread(O, flat, ctrl, n, nsteps, fl["ovr"], 200)
     assert np.mean(spread <= 1e-11) > 0.9
     assert spread.max() > 1e-6
+
+
+def test_flop_counting_matches_hand_counts():
+    """The counting scalar behind every FLOP figure (oracle/orc_count.h) against operation counts made by
+    hand from the reference source, so a dropped or double-counted operation would show.
+
+    computeVehRHS (dof18.cpp:1019-1078), per statement (add, mul, div): mt (2, 1, 0), hrc (2, 2, 1),
+    E1 (4, 2, 0), E2 (9, 8, 2), E3 (5, 8, 0), A1 (1, 2, 0), A2 (1, 2, 0), A3 (0, 1, 0), udot (7, 10, 1),
+    denom (4, 11, 0), vdot / wxdot / wzdot (5, 12, 1) each: 50 add, 83 mul, 7 div.
+    tmxy_combined (dof18.cpp:247-288): c = 2 fm / sm is (0, 1, 1) in every branch; then
+      s < sm: p (1, 1, 1), sn (0, 0, 1), dn (2, 1, 0), f (0, 2, 1), fos (0, 0, 1)   -> 3 add, 5 mul, 5 div;
+      s > ss: fos = fs / s (0, 0, 1)                                                 -> 0 add, 1 mul, 2 div;
+      sm <= s <= sstar: a (0, 2, 3), sstar (3, 1, 1), f (3, 2, 0), fos (0, 0, 1)     -> 6 add, 6 mul, 6 div;
+      sstar < s <= ss: a, sstar as above, b (2, 1, 1), f (3, 2, 0), fos (0, 0, 1)    -> 8 add, 7 mul, 7 div."""
+    import lfdm_amd
+
+    flat = lfdm_amd.params_from_json(18, f"{DATA}/18dof/HMMWV/vehicle.json", f"{DATA}/18dof/HMMWV/tmeasy.json")
+    rec = np.zeros(75)
+    rec[60 + 2], rec[60 + 3], rec[60 + 5], rec[60 + 6], rec[60 + 7] = 10.0, 0.2, 0.1, 0.01, 0.01
+    assert O.count_veh_rhs18(flat, rec, [100, 200, 300, 400], [50, 60, 70, 80]) == (50, 83, 7, 0)
+    # sm = 0.1, fm = 1000, ss = 0.5, fs = 800, df0 = 2e4: a = 5e4, sstar = 0.11
+    args = (20000.0, 0.1, 1000.0, 0.5, 800.0)
+    assert O.count_tmxy(0.05, *args) == (3, 5, 5, 0)
+    assert O.count_tmxy(0.8, *args) == (0, 1, 2, 0)
+    assert O.count_tmxy(0.105, *args) == (6, 6, 6, 0)
+    assert O.count_tmxy(0.2, *args) == (8, 7, 7, 0)
+
+
+def test_distinct_call_tallies():
+    """The distinct-argument tallies count a libm call once per step per argument: the 24-DOF step calls
+    sin / cos of pitch, roll, yaw and steering angle dozens of times, its distinct calls a few."""
+    import lfdm_amd
+
+    d = f"{DATA}/24dof/HMMWV"
+    flat = lfdm_amd.params_from_json(24, f"{d}/vehicle.json", f"{d}/tmeasy.json", f"{d}/suspension.json")
+    ctrl = np.tile(np.array([0.1, 0.5, 0.0])[None, :, None], (100, 1, 1))
+    c = O.count_ops(24, flat, ("steps", ctrl), 100)
+    assert c["sin"] >= 70 * 100 and c["cos"] >= 70 * 100
+    assert c["sin_distinct"] <= 5 * 100 and c["cos_distinct"] <= 5 * 100
+    for f in ("sin", "cos", "tan", "atan2", "hypot", "tanh"):
+        assert c[f + "_distinct"] <= c[f]
