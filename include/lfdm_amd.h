@@ -124,6 +124,18 @@ int lfdm_set_states(lfdm_solver* h, const double* records);
  * the engine torque-map ordinates), maxBrakeTorque.  Defaults come from each vehicle's group.  A pending
  * adjoint (lfdm_adjoint_forward without its backward yet) reads them again, so changing them cancels it. */
 int lfdm_set_overrides(lfdm_solver* h, const double* ovr);
+/* A vehicle's parameters in full, as the flat vector of lfdm_params_from_json: the reference keeps one
+ * VehicleParam / TMeasyParam copy per vehicle in SimData (dof18_halfImplicit_gpu.cuh:209-215,
+ * dof18_gpu.cuh:821-838) and lets users edit any field of any vehicle.  Here vehicles with equal parameters
+ * share one parameter set (so a wavefront never mixes sets).  lfdm_get_vehicle_params returns the
+ * vehicle's set with its per-vehicle parameters applied (m, jz, jx, dfy0Pn, dfy0P2n, maxBrakeTorque, and
+ * the torque map's ordinates times the torque scale); lfdm_set_vehicle_params gives the vehicle this
+ * vector as its parameters: it joins the set with exactly these values and its own driver table, or
+ * gets a set of its own, its per-vehicle parameters become the vector's, and the slot layout is rebuilt
+ * at the next launch (records kept; a pending adjoint is cancelled).  Many vehicles with distinct sets
+ * cost one wavefront each. */
+int lfdm_get_vehicle_params(lfdm_solver* h, unsigned vehicle, double* flat, int cap, int* len);
+int lfdm_set_vehicle_params(lfdm_solver* h, unsigned vehicle, const double* flat, int len);
 /* Device-pointer forms of lfdm_set_overrides and lfdm_set_states (device_ovr[7][total],
  * device_records[total][record_len]): copied on the handle's stream without synchronizing and without
  * passing through the host (e.g. from torch tensors every optimization step); values are not checked. */
@@ -138,10 +150,20 @@ int lfdm_set_controls_knots(lfdm_solver* h, int ns, double ds, int np, double dp
 int lfdm_set_controls_steps(lfdm_solver* h, int nsteps, const double* ctrl);
 int lfdm_set_controls_steps_device(lfdm_solver* h, int nsteps, const double* device_ctrl);
 int lfdm_set_controls_table(lfdm_solver* h);
-/* Per-step controls shared by every vehicle, device_ctrl[nsteps][3] (steering, throttle, braking):
- * what the reference's templated SolveStep(t, functor) evaluates inside its kernel
- * (dof18_halfImplicit_gpu.cuh:233-290, 454-551); lfdm_amd_functor.hpp fills it from a functor. */
+/* Per-step controls shared by every vehicle, device_ctrl[nsteps][3] (steering, throttle, braking), e.g.
+ * a controller's command for a whole fleet. */
 int lfdm_set_controls_shared_device(lfdm_solver* h, int nsteps, const double* device_ctrl);
+/* SolveStep(t, functor) of the reference (dof18_halfImplicit_gpu.cuh:233-290, kernels :454-640): the
+ * controls functor runs on every vehicle's lane at every step, so the integration kernel is
+ * instantiated on the functor's type in the caller's translation unit (include/lfdm_amd_functor.hpp).
+ * lfdm_functor_chunk_begin prepares the handle's device image, sets its time base to t and writes the
+ * launch record of nsteps steps (record_size: the caller's sizeof of the library's record type, which
+ * must match), the kernel's block and grid sizes and the handle's stream; the caller launches its kernel
+ * on that stream, then calls lfdm_functor_chunk_done(h, nsteps), which advances the time base.  No
+ * synchronization; refused while CSV output is set (the reference exits in that case). */
+int lfdm_functor_chunk_begin(lfdm_solver* h, double t, int nsteps, void* record, int record_size, int* block,
+                             int* grid, void** stream);
+int lfdm_functor_chunk_done(lfdm_solver* h, int nsteps);
 /* Number of steps one kernel-sim-time chunk starting at t takes (the reference kernel's loop
  * t < t0 + kernel_sim_time - step/10, dof18_halfImplicit_gpu.cu:892-948). */
 int lfdm_chunk_steps(lfdm_solver* h, double t, int* nsteps);

@@ -41,6 +41,105 @@ inline void check(int rc) {
     if (rc < 0) throw Error(rc, lfdm_last_error());
 }
 
+// MapEntry (utils_gpu.cuh): one point of a piecewise-linear map.
+struct MapEntry {
+    double _x = 0., _y = 0.;
+};
+
+namespace detail {
+
+// Readers and writers of the flat parameter vector (the order of lfdm_params_from_json): a parameter
+// struct lists its fields once, in that order, in visit(), which both directions walk.
+struct FlatIn {
+    const std::vector<double>& v;
+    size_t i = 0;
+    double next() {
+        if (i >= v.size()) throw Error(LFDM_E_PARSE, "parameter vector too short for the parameter structs");
+        return v[i++];
+    }
+    void operator()(double& x) { x = next(); }
+    void operator()(bool& b) { b = next() != 0.0; }
+    void operator()(std::vector<double>& g) {
+        g.resize((size_t)next());
+        for (double& x : g) x = next();
+    }
+    void operator()(std::vector<MapEntry>& m) {
+        m.resize((size_t)next());
+        for (MapEntry& e : m) {
+            e._x = next();
+            e._y = next();
+        }
+    }
+};
+
+struct FlatOut {
+    std::vector<double> v;
+    void operator()(const double& x) { v.push_back(x); }
+    void operator()(const bool& b) { v.push_back(b ? 1.0 : 0.0); }
+    void operator()(const std::vector<double>& g) {
+        v.push_back((double)g.size());
+        v.insert(v.end(), g.begin(), g.end());
+    }
+    void operator()(const std::vector<MapEntry>& m) {
+        v.push_back((double)m.size());
+        for (const MapEntry& e : m) {
+            v.push_back(e._x);
+            v.push_back(e._y);
+        }
+    }
+};
+
+// The fields every model's VehicleParam has after its scalars (dof18_gpu.cuh:601-708): the steering
+// mode and the powertrain switches, gear ratios and the six maps (shift: x downshift, y upshift speed).
+struct PowertrainParam {
+    bool _nonLinearSteer = false, _tcbool = false, _throttleMod = false, _driveType = false, _whichWheels = false;
+    std::vector<double> _gearRatios;
+    std::vector<MapEntry> _shiftMap, _steerMap, _powertrainMap, _lossesMap, _CFmap, _TRmap;
+    template <class V>
+    void visit_powertrain(V& f) {
+        f(_nonLinearSteer); f(_tcbool); f(_throttleMod); f(_driveType); f(_whichWheels); f(_gearRatios);
+        f(_shiftMap); f(_steerMap); f(_powertrainMap); f(_lossesMap); f(_CFmap); f(_TRmap);
+    }
+};
+
+}  // namespace detail
+
+// TMeasyParam (dof18_gpu.cuh:25-148; the same fields in every model), after tireInit.
+struct TMeasyParam {
+    double _jw = 0., _rr = 0., _mu = 0., _r0 = 0., _pn = 0., _pnmax = 0., _cx = 0., _cy = 0., _kt = 0., _dx = 0.,
+           _dy = 0., _rdyncoPn = 0., _rdyncoP2n = 0., _fzRdynco = 0., _rdyncoCrit = 0., _dfx0Pn = 0., _dfx0P2n = 0.,
+           _fxmPn = 0., _fxmP2n = 0., _fxsPn = 0., _fxsP2n = 0., _sxmPn = 0., _sxmP2n = 0., _sxsPn = 0., _sxsP2n = 0.,
+           _dfy0Pn = 0., _dfy0P2n = 0., _fymPn = 0., _fymP2n = 0., _fysPn = 0., _fysP2n = 0., _symPn = 0.,
+           _symP2n = 0., _sysPn = 0., _sysP2n = 0.;
+    template <class V>
+    void visit(V& f) {
+        for (double* p : {&_jw, &_rr, &_mu, &_r0, &_pn, &_pnmax, &_cx, &_cy, &_kt, &_dx, &_dy, &_rdyncoPn,
+                          &_rdyncoP2n, &_fzRdynco, &_rdyncoCrit, &_dfx0Pn, &_dfx0P2n, &_fxmPn, &_fxmP2n, &_fxsPn,
+                          &_fxsP2n, &_sxmPn, &_sxmP2n, &_sxsPn, &_sxsP2n, &_dfy0Pn, &_dfy0P2n, &_fymPn, &_fymP2n,
+                          &_fysPn, &_fysP2n, &_symPn, &_symP2n, &_sysPn, &_sysP2n})
+            f(*p);
+    }
+};
+
+// TMeasyNrParam (dof18_gpu.cuh:255-399), after tireInit.
+struct TMeasyNrParam {
+    double _jw = 0., _rr = 0., _mu = 0., _r0 = 0., _width = 0., _rim_radius = 0., _pn = 0., _pnmax = 0., _kt = 0.,
+           _rdyncoPn = 0., _rdyncoP2n = 0., _fzRdynco = 0., _rdyncoCrit = 0., _dfx0Pn = 0., _dfx0P2n = 0.,
+           _fxmPn = 0., _fxmP2n = 0., _fxsPn = 0., _fxsP2n = 0., _sxmPn = 0., _sxmP2n = 0., _sxsPn = 0., _sxsP2n = 0.,
+           _dfy0Pn = 0., _dfy0P2n = 0., _fymPn = 0., _fymP2n = 0., _fysPn = 0., _fysP2n = 0., _symPn = 0.,
+           _symP2n = 0., _sysPn = 0., _sysP2n = 0., _vcoulomb = 0., _frblend_begin = 0., _frblend_end = 0.,
+           _bearingCapacity = 0., _li = 0., _p_li = 0., _p_use = 0.;
+    template <class V>
+    void visit(V& f) {
+        for (double* p : {&_jw, &_rr, &_mu, &_r0, &_width, &_rim_radius, &_pn, &_pnmax, &_kt, &_rdyncoPn,
+                          &_rdyncoP2n, &_fzRdynco, &_rdyncoCrit, &_dfx0Pn, &_dfx0P2n, &_fxmPn, &_fxmP2n, &_fxsPn,
+                          &_fxsP2n, &_sxmPn, &_sxmP2n, &_sxsPn, &_sxsP2n, &_dfy0Pn, &_dfy0P2n, &_fymPn, &_fymP2n,
+                          &_fysPn, &_fysP2n, &_symPn, &_symP2n, &_sysPn, &_sysP2n, &_vcoulomb, &_frblend_begin,
+                          &_frblend_end, &_bearingCapacity, &_li, &_p_li, &_p_use})
+            f(*p);
+    }
+};
+
 
 namespace d18GPU {
 
@@ -237,6 +336,23 @@ struct SimStateNr {
     }
 };
 
+// VehicleParam (dof18_gpu.cuh:502-810); maps and gear ratios as vectors instead of device pointers and sizes.
+struct VehicleParam : detail::PowertrainParam {
+    double _a = 0., _b = 0., _h = 0., _m = 0., _jz = 0., _jx = 0., _jxz = 0., _cf = 0., _cr = 0., _muf = 0.,
+           _mur = 0., _hrcf = 0., _hrcr = 0., _krof = 0., _kror = 0., _brof = 0., _bror = 0., _maxSteer =
+           0., _crankInertia = 0., _maxBrakeTorque = 0.;
+    template <class V>
+    void visit(V& f) {
+        for (double* p : {&_a, &_b, &_h, &_m, &_jz, &_jx, &_jxz, &_cf, &_cr, &_muf, &_mur, &_hrcf,
+                          &_hrcr, &_krof, &_kror, &_brof, &_bror, &_maxSteer, &_crankInertia,
+                          &_maxBrakeTorque})
+            f(*p);
+        visit_powertrain(f);
+    }
+};
+using lfdm_amd::TMeasyNrParam;
+using lfdm_amd::TMeasyParam;
+
 }  // namespace d18GPU
 
 namespace d11GPU {
@@ -319,6 +435,21 @@ struct SimStateNr {
         _veh_state.from_record(r + k);
     }
 };
+
+// VehicleParam (dof11_gpu.cuh:496-620); maps and gear ratios as vectors instead of device pointers and sizes.
+struct VehicleParam : detail::PowertrainParam {
+    double _a = 0., _b = 0., _h = 0., _m = 0., _jz = 0., _muf = 0., _mur = 0., _maxSteer = 0., _crankInertia
+           = 0., _maxBrakeTorque = 0.;
+    template <class V>
+    void visit(V& f) {
+        for (double* p : {&_a, &_b, &_h, &_m, &_jz, &_muf, &_mur, &_maxSteer, &_crankInertia,
+                          &_maxBrakeTorque})
+            f(*p);
+        visit_powertrain(f);
+    }
+};
+using lfdm_amd::TMeasyNrParam;
+using lfdm_amd::TMeasyParam;
 
 }  // namespace d11GPU
 
@@ -697,6 +828,31 @@ struct SimStateNr {
     }
 };
 
+// VehicleParam (dof24_gpu.cuh:580-764); maps and gear ratios as vectors instead of device pointers and sizes.
+struct VehicleParam : detail::PowertrainParam {
+    double _a = 0., _b = 0., _h = 0., _m = 0., _jz = 0., _jx = 0., _jy = 0., _cf = 0., _cr = 0., _muf = 0.,
+           _mur = 0., _hrcf = 0., _hrcr = 0., _maxSteer = 0., _crankInertia = 0., _maxBrakeTorque = 0.;
+    template <class V>
+    void visit(V& f) {
+        for (double* p : {&_a, &_b, &_h, &_m, &_jz, &_jx, &_jy, &_cf, &_cr, &_muf, &_mur, &_hrcf,
+                          &_hrcr, &_maxSteer, &_crankInertia, &_maxBrakeTorque})
+            f(*p);
+        visit_powertrain(f);
+    }
+};
+using lfdm_amd::TMeasyNrParam;
+using lfdm_amd::TMeasyParam;
+
+// SuspensionParam (dof24_gpu.cuh:950-955)
+struct SuspensionParam {
+    double _ks = 0., _bs = 0.;
+    template <class V>
+    void visit(V& f) {
+        f(_ks);
+        f(_bs);
+    }
+};
+
 }  // namespace d24GPU
 
 
@@ -751,8 +907,31 @@ class SolverBase {
     }
     // The C-ABI handle, for the batched extensions of lfdm_amd.h.
     lfdm_solver* handle() const { return h_; }
+    // One vehicle's parameters as the flat vector of lfdm_params_from_json (lfdm_get_vehicle_params).
+    std::vector<double> GetParamsFlat(unsigned vehicle) const {
+        std::vector<double> v(8192);
+        int n = 0;
+        check(lfdm_get_vehicle_params(h_, vehicle, v.data(), (int)v.size(), &n));
+        v.resize((size_t)n);
+        return v;
+    }
+    void SetParamsFlat(unsigned vehicle, const std::vector<double>& flat) {
+        check(lfdm_set_vehicle_params(h_, vehicle, flat.data(), (int)flat.size()));
+    }
 
   protected:
+    template <class... P>
+    void get_params(unsigned vehicle, P&... parts) const {
+        const std::vector<double> flat = GetParamsFlat(vehicle);
+        FlatIn in{flat};
+        (parts.visit(in), ...);
+    }
+    template <class... P>
+    void set_params(unsigned vehicle, P... parts) {  // copies: visit() walks the fields of a mutable object
+        FlatOut out;
+        (parts.visit(out), ...);
+        SetParamsFlat(vehicle, out.v);
+    }
     static int check_len(int n) {
         check(n);
         return n;
@@ -793,6 +972,21 @@ class d18SolverHalfImplicitGPU : public detail::SolverBase {
     d18GPU::SimState GetSimState(unsigned int vehicle_index) const { return get<d18GPU::SimState>(vehicle_index); }
     d18GPU::SimStateNr GetSimStateNr(unsigned int vehicle_index) const {
         return get<d18GPU::SimStateNr>(vehicle_index);
+    }
+
+    // A vehicle's parameters, as the reference's m_sim_data[vehicle] holds them (the per-vehicle
+    // parameters applied); SetParams gives one vehicle its own (any field).
+    void GetParams(unsigned vehicle, d18GPU::VehicleParam& veh, d18GPU::TMeasyParam& tire) const {
+        get_params(vehicle, veh, tire);
+    }
+    void GetParams(unsigned vehicle, d18GPU::VehicleParam& veh, d18GPU::TMeasyNrParam& tire) const {
+        get_params(vehicle, veh, tire);
+    }
+    void SetParams(unsigned vehicle, const d18GPU::VehicleParam& veh, const d18GPU::TMeasyParam& tire) {
+        set_params(vehicle, veh, tire);
+    }
+    void SetParams(unsigned vehicle, const d18GPU::VehicleParam& veh, const d18GPU::TMeasyNrParam& tire) {
+        set_params(vehicle, veh, tire);
     }
 
   private:
@@ -838,6 +1032,21 @@ class d11SolverHalfImplicitGPU : public detail::SolverBase {
     d11GPU::SimState GetSimState(unsigned int vehicle_index) const { return get<d11GPU::SimState>(vehicle_index); }
     d11GPU::SimStateNr GetSimStateNr(unsigned int vehicle_index) const {
         return get<d11GPU::SimStateNr>(vehicle_index);
+    }
+
+    // A vehicle's parameters, as the reference's m_sim_data[vehicle] holds them (the per-vehicle
+    // parameters applied); SetParams gives one vehicle its own (any field).
+    void GetParams(unsigned vehicle, d11GPU::VehicleParam& veh, d11GPU::TMeasyParam& tire) const {
+        get_params(vehicle, veh, tire);
+    }
+    void GetParams(unsigned vehicle, d11GPU::VehicleParam& veh, d11GPU::TMeasyNrParam& tire) const {
+        get_params(vehicle, veh, tire);
+    }
+    void SetParams(unsigned vehicle, const d11GPU::VehicleParam& veh, const d11GPU::TMeasyParam& tire) {
+        set_params(vehicle, veh, tire);
+    }
+    void SetParams(unsigned vehicle, const d11GPU::VehicleParam& veh, const d11GPU::TMeasyNrParam& tire) {
+        set_params(vehicle, veh, tire);
     }
 
   private:
@@ -888,6 +1097,21 @@ class d24SolverHalfImplicitGPU : public detail::SolverBase {
     d24GPU::SimState GetSimState(unsigned int vehicle_index) const { return get<d24GPU::SimState>(vehicle_index); }
     d24GPU::SimStateNr GetSimStateNr(unsigned int vehicle_index) const {
         return get<d24GPU::SimStateNr>(vehicle_index);
+    }
+
+    // A vehicle's parameters, as the reference's m_sim_data[vehicle] holds them (the per-vehicle
+    // parameters applied); SetParams gives one vehicle its own (any field).
+    void GetParams(unsigned vehicle, d24GPU::VehicleParam& veh, d24GPU::TMeasyParam& tire, d24GPU::SuspensionParam& sus) const {
+        get_params(vehicle, veh, tire, sus);
+    }
+    void GetParams(unsigned vehicle, d24GPU::VehicleParam& veh, d24GPU::TMeasyNrParam& tire, d24GPU::SuspensionParam& sus) const {
+        get_params(vehicle, veh, tire, sus);
+    }
+    void SetParams(unsigned vehicle, const d24GPU::VehicleParam& veh, const d24GPU::TMeasyParam& tire, const d24GPU::SuspensionParam& sus) {
+        set_params(vehicle, veh, tire, sus);
+    }
+    void SetParams(unsigned vehicle, const d24GPU::VehicleParam& veh, const d24GPU::TMeasyNrParam& tire, const d24GPU::SuspensionParam& sus) {
+        set_params(vehicle, veh, tire, sus);
     }
 
   private:

@@ -1,66 +1,105 @@
 // SolveStep(t, functor) of the reference GPU solvers (dof18_halfImplicit_gpu.cuh:233-290 and the
-// templated Integrate kernels :454-640): the functor `__device__ void operator()(double t,
-// DriverInput* controls)` supplies the controls of every step of one kernel-sim-time chunk.
+// templated Integrate kernels :454-640; the 11/24-DOF twins alike): the functor
+// `__device__ void operator()(double t, DriverInput* controls)` supplies the controls of every vehicle at
+// every step of one kernel-sim-time chunk, called on the vehicle's own lane with the step's time, so it may
+// read blockIdx / threadIdx (slot = blockIdx.x * blockDim.x + threadIdx.x; with one Construct group the slot
+// is the vehicle index, as in the reference) or per-vehicle device data.  A functor may also take the
+// vehicle index itself: `operator()(double t, DriverInput* controls, unsigned vehicle)`, which stays the
+// vehicle's index whatever the slot layout (several Construct groups, horizon layout).
 //
-// Needs a HIP compiler (hipcc).  The functor is evaluated on the device at each step's time (t
-// accumulated from the chunk start by `t += step`, as the integration itself does) into a
-// [steps][3] control table that the integration kernel then reads for the whole fleet.
+// Needs a HIP compiler (hipcc): the library's integration kernel is instantiated here on the functor's type
+// and launched on the handle's stream with the launch record the library prepares
+// (lfdm_functor_chunk_begin / _done); nothing is allocated and nothing synchronizes.  Compile with
+// -ffp-contract=off, as the library is, so the step keeps the reference's operation order.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "../low-fidelity-dynamic-models_amd/csrc/integrate.h"
 #include "lfdm_amd.hpp"
 
 namespace lfdm_amd {
 namespace detail {
 
+template <class Solver>
+struct ModelOf;
+template <>
+struct ModelOf<d18SolverHalfImplicitGPU> {
+    using type = lfdm::kern::M18;
+};
+template <>
+struct ModelOf<d11SolverHalfImplicitGPU> {
+    using type = lfdm::kern::M11;
+};
+template <>
+struct ModelOf<d24SolverHalfImplicitGPU> {
+    using type = lfdm::kern::M24;
+};
+
+// Calls the functor with the vehicle index where it takes one (overload resolution in device code, where a
+// __device__ call operator is visible; the int argument prefers this form).
 template <class Func>
-__global__ void eval_functor_controls(Func func, double t0, double step, int nsteps, double* ctrl) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    double t = t0;
-    for (int k = 0; k < nsteps; ++k) {
-        DriverInput in;
-        in.m_time = t;
-        func(t, &in);
-        ctrl[3 * k + 0] = in.m_steering;
-        ctrl[3 * k + 1] = in.m_throttle;
-        ctrl[3 * k + 2] = in.m_braking;
-        t = t + step;
-    }
+__device__ auto call_functor(Func& f, double t, DriverInput* in, unsigned vehicle, int)
+    -> decltype(f(t, in, vehicle), void()) {
+    f(t, in, vehicle);
+}
+template <class Func>
+__device__ void call_functor(Func& f, double t, DriverInput* in, unsigned, long) {
+    f(t, in);
 }
 
-// Device table reused across calls of one process (grown on demand).
-struct FunctorTable {
-    double* ptr = nullptr;
-    int cap = 0;
-    ~FunctorTable() {
-        if (ptr) (void)hipFree(ptr);
+// The kernel's control source around the user's functor.
+template <class Func>
+struct FunctorControls {
+    Func func;
+    __device__ void eval(double t, int vehicle, double& s, double& th, double& b) {
+        DriverInput in;
+        in.m_time = t;
+        call_functor(func, t, &in, (unsigned)vehicle, 0);
+        s = in.m_steering;
+        th = in.m_throttle;
+        b = in.m_braking;
     }
 };
 
+template <class M, bool NR, int W, bool STRICT, class F>
+hipError_t launch_functor_at(const lfdm::Launch& L, int block, int grid, hipStream_t st, const F& f) {
+    hipLaunchKernelGGL((lfdm::kern::integrate<M, NR, lfdm::kCtrlFunctor, W, STRICT, F>), dim3(grid), dim3(block), 0,
+                       st, L, f);
+    return hipGetLastError();
+}
+
+// The library's choice of kernel (lfdm::kern::launch_model): the strict step at the model's default
+// occupancy; the fast step at one wave per SIMD for fleets of at most one wavefront per SIMD, else at the
+// model's default.
+template <class M, bool NR, class F>
+hipError_t launch_functor_tire(const lfdm::Launch& L, int block, int grid, hipStream_t st, const F& f) {
+    if (L.strict) return launch_functor_at<M, NR, M::kWaves, true>(L, block, grid, st, f);
+    if (L.waves == 1) return launch_functor_at<M, NR, 1, false>(L, block, grid, st, f);
+    return launch_functor_at<M, NR, M::kWaves, false>(L, block, grid, st, f);
+}
+
 }  // namespace detail
 
-// One chunk with controls from `func`; returns t + kernel_sim_time like the reference.  The solver
-// must not have output enabled (the reference exits in that case; here it throws).
+// One chunk with controls from `func`; returns t + kernel_sim_time like the reference.  The solver must
+// not have CSV output set (the reference exits in that case; here it throws).
 template <class Solver, class Func>
 double SolveStep(Solver& solver, double t, Func func) {
-    static detail::FunctorTable table;
+    using M = typename detail::ModelOf<Solver>::type;
     lfdm_solver* h = solver.handle();
-    check(lfdm_synchronize(h));  // the previous chunk may still read the table
     int n = 0;
     check(lfdm_chunk_steps(h, t, &n));
-    if (n > table.cap) {
-        if (table.ptr) (void)hipFree(table.ptr);
-        table.ptr = nullptr;
-        if (hipMalloc((void**)&table.ptr, (size_t)n * 3 * sizeof(double)) != hipSuccess)
-            throw Error(LFDM_E_HIP, "cannot allocate the functor control table");
-        table.cap = n;
+    lfdm::Launch L;
+    int block = 0, grid = 0;
+    void* stream = nullptr;
+    check(lfdm_functor_chunk_begin(h, t, n, &L, (int)sizeof(L), &block, &grid, &stream));
+    if (n > 0 && L.S > 0) {
+        const detail::FunctorControls<Func> f{func};
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        const hipError_t e = L.nr ? detail::launch_functor_tire<M, true>(L, block, grid, st, f)
+                                  : detail::launch_functor_tire<M, false>(L, block, grid, st, f);
+        if (e != hipSuccess) throw Error(LFDM_E_HIP, std::string("functor integration launch: ") + hipGetErrorString(e));
     }
-    hipLaunchKernelGGL(detail::eval_functor_controls<Func>, dim3(1), dim3(64), 0, 0, func, t, solver.GetStep(), n,
-                       table.ptr);
-    if (hipDeviceSynchronize() != hipSuccess) throw Error(LFDM_E_HIP, "functor control evaluation failed");
-    check(lfdm_set_controls_shared_device(h, n, table.ptr));
-    check(lfdm_set_time(h, t));
-    check(lfdm_advance(h, n));
+    check(lfdm_functor_chunk_done(h, n));
     double ksim = 0.0;
     check(lfdm_get_kernel_sim_time(h, &ksim));
     return t + ksim;

@@ -24,6 +24,13 @@ struct Ctl {
     double s, th, b;
 };
 
+// Controls of kCtrlFunctor launches come from a device functor F with
+//   __device__ void eval(double t, int vehicle, double& steering, double& throttle, double& braking);
+// the library's own launches (every other control source) carry this empty one.
+struct NoControls {
+    __device__ void eval(double, int, double&, double&, double&) {}
+};
+
 template <int MODE>
 struct CtrlState {
     int kcur = -1;
@@ -31,11 +38,13 @@ struct CtrlState {
 };
 
 // Per-vehicle control buffers are indexed by user vehicle index `uv` with stride n_user.
-template <int MODE>
+template <int MODE, class F>
 __device__ __forceinline__ Ctl controls_at(const Launch& L, const PSet& P, CtrlState<MODE>& cs, int uv, int k,
-                                           double t) {
+                                           double t, F& f) {
     Ctl c;
-    if (MODE == kCtrlConst) {
+    if constexpr (MODE == kCtrlFunctor) {
+        f.eval(t, uv, c.s, c.th, c.b);
+    } else if (MODE == kCtrlConst) {
         c.s = L.c_s; c.th = L.c_t; c.b = L.c_b;
     } else if (MODE == kCtrlTable) {
         driver_input(t, L.drv + P.drv_off, P.drv_rows, c.s, c.th, c.b);
@@ -337,8 +346,8 @@ struct M24 {
 // Launch::waves picks (0: the model's default).  STRICT instantiations evaluate the step with the
 // reference's own quotients and transcendentals (Launch::strict; built at the model's default
 // occupancy only).
-template <class M, bool NR, int MODE, int W, bool STRICT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) void integrate(Launch L) {
+template <class M, bool NR, int MODE, int W, bool STRICT, class F = NoControls>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) void integrate(Launch L, F f) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int wave = uniform(slot >> 6);
     if (wave * kWave >= L.S) return;
@@ -373,7 +382,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
     // all steps but the lane's last keep only the carried state ...
     for (int kk = 0; kk < wsteps - 1; ++kk) {
         if (kk < my - 1) {
-            const Ctl c = controls_at<MODE>(L, P, cs, uv, kk, t);
+            const Ctl c = controls_at<MODE>(L, P, cs, uv, kk, t, f);
             M::template step<NR, false, STRICT>(P, u, k, s, c, dt, nullptr);
             t = t + dt;
         }
@@ -389,7 +398,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
     if (my > 0) {
         typename M::X x;
         CtrlState<MODE> cl;
-        const Ctl c = controls_at<MODE>(L, P, my == wsteps ? cs : cl, uv, my - 1, t);
+        const Ctl c = controls_at<MODE>(L, P, my == wsteps ? cs : cl, uv, my - 1, t, f);
         M::template step<NR, true, STRICT>(P, u, k, s, c, dt, &x);
         t = t + dt;
         M::template store<NR>(L.state, S, slot, P, s, x);
@@ -411,10 +420,10 @@ template <class M, bool NR, int W, bool STRICT>
 hipError_t launch_mode(const Launch& L, int block, hipStream_t stream) {
     const int grid = (L.S + block - 1) / block;
     switch (L.mode) {
-        case kCtrlTable: hipLaunchKernelGGL((integrate<M, NR, kCtrlTable, W, STRICT>), grid, block, 0, stream, L); break;
-        case kCtrlConst: hipLaunchKernelGGL((integrate<M, NR, kCtrlConst, W, STRICT>), grid, block, 0, stream, L); break;
-        case kCtrlKnots: hipLaunchKernelGGL((integrate<M, NR, kCtrlKnots, W, STRICT>), grid, block, 0, stream, L); break;
-        case kCtrlSteps: hipLaunchKernelGGL((integrate<M, NR, kCtrlSteps, W, STRICT>), grid, block, 0, stream, L); break;
+        case kCtrlTable: hipLaunchKernelGGL((integrate<M, NR, kCtrlTable, W, STRICT>), grid, block, 0, stream, L, NoControls{}); break;
+        case kCtrlConst: hipLaunchKernelGGL((integrate<M, NR, kCtrlConst, W, STRICT>), grid, block, 0, stream, L, NoControls{}); break;
+        case kCtrlKnots: hipLaunchKernelGGL((integrate<M, NR, kCtrlKnots, W, STRICT>), grid, block, 0, stream, L, NoControls{}); break;
+        case kCtrlSteps: hipLaunchKernelGGL((integrate<M, NR, kCtrlSteps, W, STRICT>), grid, block, 0, stream, L, NoControls{}); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -74,6 +74,7 @@ enum CtrlMode : int {
     kCtrlConst = 1,  // one (steering, throttle, braking) for every vehicle (SolveStep)
     kCtrlKnots = 2,  // per-vehicle knot program (steering lerp grid, throttle/brake step grid)
     kCtrlSteps = 3,  // per-vehicle per-step arrays [step][3][slot]
+    kCtrlFunctor = 4,  // a device functor evaluated per vehicle and step (kernel instantiated by the caller)
 };
 
 }  // namespace lfdm

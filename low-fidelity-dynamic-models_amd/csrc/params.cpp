@@ -476,6 +476,29 @@ bool pset_from_flat(int dof, const double* flat, int len, int tire_nr, PSet& o, 
     return true;
 }
 
+bool flat_slots(int dof, const double* flat, int len, int tire_nr, FlatSlots& o, std::string& err) {
+    PSet check;
+    if (!pset_from_flat(dof, flat, len, tire_nr, check, err)) return false;  // validates the layout
+    const int nscal = dof == 11 ? 10 : (dof == 18 ? 20 : 16);
+    o.m = 3;
+    o.jz = 4;
+    o.jx = dof == 11 ? -1 : 5;
+    o.max_brake = nscal - 1;
+    int i = nscal + 5;  // after the five flags: the gear count
+    i += 1 + (int)flat[i];
+    for (int map = 0; map < 6; ++map) {  // shift, steer, torque, losses, capacity factor, torque ratio
+        const int n = (int)flat[i];
+        if (map == 2) {
+            o.torque = i + 1;
+            o.torque_n = n;
+        }
+        i += 1 + 2 * n;
+    }
+    o.dfy0Pn = i + (tire_nr ? 23 : 25);
+    o.dfy0P2n = o.dfy0Pn + 1;
+    return true;
+}
+
 bool load_driver_file(const std::string& path, std::vector<DrvRow>& rows, std::string& err) {
     std::ifstream f(path);
     if (!f) {

@@ -32,6 +32,13 @@ bool params_from_json(int dof, const std::string& veh, const std::string& tire, 
                       std::vector<double>& flat, std::string& err);
 bool pset_from_flat(int dof, const double* flat, int len, int tire_nr, PSet& out, std::string& err);
 
+// Positions in a flat parameter vector of the values the per-vehicle parameters stand for (jx is -1 for
+// the 11-DOF model, which has no roll inertia; the torque map's ordinates are flat[torque + 2 k + 1]).
+struct FlatSlots {
+    int m, jz, jx, max_brake, dfy0Pn, dfy0P2n, torque, torque_n;
+};
+bool flat_slots(int dof, const double* flat, int len, int tire_nr, FlatSlots& out, std::string& err);
+
 // initializeTireSus (dof24.cpp:34-89) on one 24-DOF record, with the vehicle's own mass (which
 // may differ from the set's through the per-vehicle parameters).
 void init_tire_sus24(const PSet& p, double m, int tire_nr, double* rec);

@@ -69,8 +69,8 @@ struct Group {
     int tire_nr = 0;
     PSet pset;
     std::vector<DrvRow> rows;
-    unsigned n = 0;
-    unsigned user_base = 0;
+    std::vector<int> users;  // vehicles of this set, ascending (a Construct call's range, less vehicles given
+                             // parameters of their own, plus the vehicles given exactly this set)
 };
 
 template <class T>
@@ -86,6 +86,7 @@ struct lfdm_solver {
     unsigned total = 0;
     int tire_type = -1;
     std::vector<Group> groups;
+    std::vector<int> group_of;  // parameter set of every constructed vehicle
     unsigned n_params = 0;
     std::vector<double> init;  // [n_states][rec]
     unsigned n_states = 0;
@@ -250,7 +251,7 @@ int ensure_device(lfdm_solver* h) {
     std::vector<int> slot_base(h->groups.size());
     for (size_t g = 0; g < h->groups.size(); ++g) {
         slot_base[g] = S;
-        S += (int)((h->groups[g].n + kWave - 1) / kWave) * kWave;
+        S += (int)((h->groups[g].users.size() + kWave - 1) / kWave) * kWave;
     }
     h->S = S;
     std::vector<int> slot_user(S, -1), wave_set(S / kWave, 0);
@@ -260,16 +261,15 @@ int ensure_device(lfdm_solver* h) {
     const bool by_horizon = h->layout == LFDM_LAYOUT_BY_HORIZON && !h->horizon_host.empty();
     for (size_t g = 0; g < h->groups.size(); ++g) {
         const Group& G = h->groups[g];
-        std::vector<int> order(G.n);
-        std::iota(order.begin(), order.end(), (int)G.user_base);
+        std::vector<int> order(G.users);
         if (by_horizon)  // longest horizon first: lanes of a wavefront stop together, long wavefronts start first
             std::stable_sort(order.begin(), order.end(),
                              [&](int a, int b) { return h->horizon_host[a] > h->horizon_host[b]; });
-        for (unsigned i = 0; i < G.n; ++i) {
+        for (size_t i = 0; i < order.size(); ++i) {
             slot_user[slot_base[g] + i] = order[i];
             h->user_slot[order[i]] = slot_base[g] + (int)i;
         }
-        const int waves = (int)((G.n + kWave - 1) / kWave);
+        const int waves = (int)((order.size() + kWave - 1) / kWave);
         for (int w = 0; w < waves; ++w) wave_set[slot_base[g] / kWave + w] = (int)g;
         sets[g] = G.pset;
         sets[g].drv_off = (int)pool.size();
@@ -283,7 +283,7 @@ int ensure_device(lfdm_solver* h) {
         int u = slot_user[s];
         if (u < 0) {
             int g = wave_set[s / kWave];
-            u = (int)h->groups[g].user_base;
+            u = h->groups[g].users[0];
         }
         for (int f = 0; f < rec; ++f) st[(size_t)f * S + s] = h->init[(size_t)u * rec + f];
     }
@@ -606,9 +606,9 @@ std::vector<Launch> run_launches(lfdm_solver* h, int nsteps, int spl, int waves,
 double predicted_lane_steps(const lfdm_solver* h, int layout, int nsteps) {
     double sum = 0;
     for (const Group& G : h->groups) {
-        std::vector<int> hz(G.n, nsteps);
+        std::vector<int> hz(G.users.size(), nsteps);
         if (!h->horizon_host.empty())
-            for (unsigned i = 0; i < G.n; ++i) hz[i] = std::min(h->horizon_host[G.user_base + i], nsteps);
+            for (size_t i = 0; i < G.users.size(); ++i) hz[i] = std::min(h->horizon_host[G.users[i]], nsteps);
         if (layout == LFDM_LAYOUT_BY_HORIZON) std::stable_sort(hz.begin(), hz.end(), std::greater<int>());
         for (size_t w = 0; w < hz.size(); w += kWave) {
             int m = 0;
@@ -718,8 +718,9 @@ int lfdm_construct_flat(lfdm_solver* h, const double* flat, int len, int tire_ty
     std::string err;
     if (!pset_from_flat(h->dof, flat, len, tire_type, g.pset, err)) return fail(LFDM_E_ARG, err);
     if (rows && nrows > 0) g.rows.assign(reinterpret_cast<const DrvRow*>(rows), reinterpret_cast<const DrvRow*>(rows) + nrows);
-    g.n = n;
-    g.user_base = h->n_params;
+    g.users.resize(n);
+    std::iota(g.users.begin(), g.users.end(), (int)h->n_params);
+    h->group_of.resize(h->n_params + n, (int)h->groups.size());
     h->groups.push_back(std::move(g));
     h->n_params += n;
     h->tire_type = tire_type;
@@ -761,10 +762,8 @@ int append_states(lfdm_solver* h, const double* records, unsigned n, size_t stri
     if (h->dof == 24) {
         // initializeTireSus per vehicle (dof24_halfImplicit_gpu.cu:320-333), with the vehicle's own
         // mass when per-vehicle parameters were set before Initialize
-        size_t gi = 0;
         for (unsigned i = first; i < first + n; ++i) {
-            while (i >= h->groups[gi].user_base + h->groups[gi].n) ++gi;
-            const Group& G = h->groups[gi];
+            const Group& G = h->groups[h->group_of[i]];
             const double m = h->ovr_host.empty() ? G.pset.m : h->ovr_host[i];
             init_tire_sus24(G.pset, m, h->tire_type == LFDM_TIRE_TMEASYNR, h->init.data() + (size_t)i * h->rec);
         }
@@ -1041,6 +1040,86 @@ int lfdm_set_overrides(lfdm_solver* h, const double* ovr) {
     return LFDM_OK;
 }
 
+int lfdm_get_vehicle_params(lfdm_solver* h, unsigned vehicle, double* out, int cap, int* len) {
+    if (int rc = check(h)) return rc;
+    if (!out || !len) return fail(LFDM_E_ARG, "null output");
+    if (vehicle >= h->n_params) return fail(LFDM_E_ARG, "vehicle " + std::to_string(vehicle) + " not constructed");
+    const Group& G = h->groups[h->group_of[vehicle]];
+    const int n = (int)G.flat.size();
+    *len = n;
+    if (cap < n) return fail(LFDM_E_CAPACITY, "parameter vector needs " + std::to_string(n) + " doubles");
+    std::copy(G.flat.begin(), G.flat.end(), out);
+    double ovr[kOverrides];
+    bool have = false;
+    if (h->built && h->ovr_device_newer) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int f = 0; f < kOverrides; ++f)
+            HIPCHK(hipMemcpy(&ovr[f], h->d_ovr + (size_t)f * h->S + h->user_slot[vehicle], sizeof(double),
+                             hipMemcpyDeviceToHost));
+        have = true;
+    } else if (!h->ovr_host.empty()) {
+        for (int f = 0; f < kOverrides; ++f) ovr[f] = h->ovr_host[(size_t)f * h->total + vehicle];
+        have = true;
+    }
+    if (have) {  // the vehicle's own values of the per-vehicle parameters
+        FlatSlots fs;
+        std::string err;
+        if (!flat_slots(h->dof, out, n, G.tire_nr == LFDM_TIRE_TMEASYNR, fs, err)) return fail(LFDM_E_PARSE, err);
+        out[fs.m] = ovr[0];
+        out[fs.jz] = ovr[1];
+        if (fs.jx >= 0) out[fs.jx] = ovr[2];
+        out[fs.dfy0Pn] = ovr[3];
+        out[fs.dfy0P2n] = ovr[4];
+        for (int k = 0; k < fs.torque_n; ++k) out[fs.torque + 2 * k + 1] = out[fs.torque + 2 * k + 1] * ovr[5];
+        out[fs.max_brake] = ovr[6];
+    }
+    return LFDM_OK;
+}
+
+int lfdm_set_vehicle_params(lfdm_solver* h, unsigned vehicle, const double* flat, int len) {
+    if (int rc = check(h)) return rc;
+    if (!flat || len <= 0) return fail(LFDM_E_ARG, "empty parameter vector");
+    if (vehicle >= h->n_params) return fail(LFDM_E_ARG, "vehicle " + std::to_string(vehicle) + " not constructed");
+    PSet ps;
+    std::string err;
+    if (!pset_from_flat(h->dof, flat, len, h->tire_type, ps, err)) return fail(LFDM_E_ARG, err);
+    if (int rc = drop_device_image(h)) return rc;  // the slot layout changes
+    const int g0 = h->group_of[vehicle];
+    const std::vector<double> fv(flat, flat + len);
+    int g1 = -1;
+    for (size_t g = 0; g < h->groups.size(); ++g) {
+        const Group& G = h->groups[g];
+        if (G.flat.size() == fv.size() && std::memcmp(G.flat.data(), fv.data(), fv.size() * sizeof(double)) == 0 &&
+            G.rows.size() == h->groups[g0].rows.size() &&
+            std::memcmp(G.rows.data(), h->groups[g0].rows.data(), G.rows.size() * sizeof(DrvRow)) == 0) {
+            g1 = (int)g;
+            break;
+        }
+    }
+    if (g1 < 0) {  // a set of its own, keeping the vehicle's driver table
+        Group ng;
+        ng.flat = fv;
+        ng.tire_nr = h->tire_type;
+        ng.pset = ps;
+        ng.rows = h->groups[g0].rows;
+        h->groups.push_back(std::move(ng));
+        g1 = (int)h->groups.size() - 1;
+    }
+    if (g1 != g0) {
+        std::vector<int>& from = h->groups[g0].users;
+        from.erase(std::lower_bound(from.begin(), from.end(), (int)vehicle));
+        std::vector<int>& to = h->groups[g1].users;
+        to.insert(std::lower_bound(to.begin(), to.end(), (int)vehicle), (int)vehicle);
+        h->group_of[vehicle] = g1;
+    }
+    if (!h->ovr_host.empty()) {  // the per-vehicle parameters take the new set's values
+        const PSet& p = h->groups[g1].pset;
+        const double v[kOverrides] = {p.m, p.jz, p.jx, p.tire.dfy0Pn, p.tire.dfy0P2n, 1.0, p.maxBrakeTorque};
+        for (int f = 0; f < kOverrides; ++f) h->ovr_host[(size_t)f * h->total + vehicle] = v[f];
+    }
+    return LFDM_OK;
+}
+
 int lfdm_set_overrides_device(lfdm_solver* h, const double* device_ovr) {
     if (int rc = check(h)) return rc;
     if (!device_ovr) return fail(LFDM_E_ARG, "null overrides");
@@ -1112,6 +1191,35 @@ int lfdm_set_controls_shared_device(lfdm_solver* h, int nsteps, const double* dc
     h->steps_shared = true;
     h->steps_dirty = false;
     h->ctrl_mode = kCtrlSteps;
+    return LFDM_OK;
+}
+
+int lfdm_functor_chunk_begin(lfdm_solver* h, double t, int nsteps, void* record, int record_size, int* block,
+                             int* grid, void** stream) {
+    if (int rc = check(h)) return rc;
+    if (!record || !block || !grid || !stream || nsteps < 0) return fail(LFDM_E_ARG, "bad functor chunk arguments");
+    if (record_size != (int)sizeof(Launch))
+        return fail(LFDM_E_ARG, "launch record size " + std::to_string(record_size) + " differs from the library's " +
+                                    std::to_string(sizeof(Launch)) + " (header and library of different versions)");
+    if (h->output)
+        return fail(LFDM_E_STATE, "SolveStep with a controls functor has no CSV output; read states with GetSimState");
+    if (int rc = ensure_device(h)) return rc;
+    if (int rc = sync_inputs(h)) return rc;
+    h->t_now = t;
+    Launch L = make_launch(h, kCtrlFunctor, t, nsteps);
+    std::memcpy(record, &L, sizeof(L));
+    *block = block_size(h->tpb);
+    *grid = (h->S + *block - 1) / *block;
+    *stream = (void*)h->stream;
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    return LFDM_OK;
+}
+
+int lfdm_functor_chunk_done(lfdm_solver* h, int nsteps) {
+    if (int rc = check(h)) return rc;
+    for (int k = 0; k < nsteps; ++k) h->t_now += h->step;
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    h->timed = true;
     return LFDM_OK;
 }
 

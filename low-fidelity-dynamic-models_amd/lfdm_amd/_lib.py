@@ -88,6 +88,8 @@ SIGNATURES = {
     "lfdm_set_states": [_p, _p],
     "lfdm_set_overrides": [_p, _p],
     "lfdm_set_overrides_device": [_p, _p],
+    "lfdm_get_vehicle_params": [_p, _u, _p, _i, ctypes.POINTER(_i)],
+    "lfdm_set_vehicle_params": [_p, _u, _p, _i],
     "lfdm_set_states_device": [_p, _p],
     "lfdm_adjoint_generation": [_p, ctypes.POINTER(ctypes.c_ulonglong)],
     "lfdm_set_controls_knots": [_p, _i, _d, _i, _d, _p],

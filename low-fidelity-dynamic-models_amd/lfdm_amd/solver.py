@@ -217,6 +217,18 @@ class SolverHalfImplicitGPU:
         assert ovr.shape == (len(OVERRIDE_FIELDS), self.total)
         check(lib.lfdm_set_overrides(self._h, ovr.ctypes.data))
 
+    def get_vehicle_params(self, vehicle):
+        """The vehicle's full flat parameter vector (its set with its per-vehicle parameters applied)."""
+        buf = np.zeros(8192)
+        n = ctypes.c_int(0)
+        check(lib.lfdm_get_vehicle_params(self._h, vehicle, buf.ctypes.data, buf.size, ctypes.byref(n)))
+        return buf[: n.value].copy()
+
+    def set_vehicle_params(self, vehicle, flat):
+        """Gives one vehicle this flat parameter vector (any field), as editing SimData of the reference."""
+        f = _f64(flat)
+        check(lib.lfdm_set_vehicle_params(self._h, vehicle, f.ctypes.data, f.size))
+
     def set_overrides_device(self, device_ptr):
         """Per-vehicle parameters from device memory, (7, total) float64 (e.g. a torch tensor's data_ptr())."""
         check(lib.lfdm_set_overrides_device(self._h, ctypes.c_void_p(device_ptr)))

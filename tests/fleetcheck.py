@@ -222,18 +222,17 @@ def mixed_gpu(models, arith=ARITH_FAST, trial_steps=200):
     return fins, cfg
 
 
-def mixed_reference(O, m, every=500):
-    """Oracle final records at each vehicle's horizon (Reference.traj), normwise denominators `den` from
-    the sampled trajectory up to it, the one-ulp spread of the final records and the finiteness flips."""
-    dof, n, flat, fl = m["dof"], m["n"], m["flat"], m["fl"]
-    ctrl = ("knots", fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
-    T = int(m["horizon"].max())
+def final_reference(O, dof, flat, ctrl, n, nsteps, ovr, init_fn=lambda o: None, horizon=None, every=500,
+                    tire_nr=False):
+    """Oracle final records (Reference.traj) of n vehicles after nsteps (or each vehicle's horizon),
+    normwise denominators `den` from the trajectory sampled every `every` steps up to the end, the one-ulp
+    spread of the final records and the finiteness flips.  init_fn(ovr) gives the initial records."""
 
-    def run(ovr):
-        init = O.init24(flat, m["recs"], False, ovr) if dof == 24 else None
-        return O.run(dof, flat, ctrl, n, T, ovr=ovr, init=init, every=every, horizon=m["horizon"])
+    def run(o):
+        return O.run(dof, flat, ctrl, n, nsteps, tire_nr=tire_nr, ovr=o, init=init_fn(o), every=every,
+                     horizon=horizon)
 
-    fin, tr = run(fl["ovr"])
+    fin, tr = run(ovr)
     with np.errstate(invalid="ignore"):
         den = np.maximum(np.nan_to_num(np.abs(tr), nan=0.0).max(axis=0), np.nan_to_num(np.abs(fin), nan=0.0))
     nf = ~np.isfinite(fin).all(axis=-1)
@@ -241,7 +240,7 @@ def mixed_reference(O, m, every=500):
     nf_flip = np.zeros(n, dtype=bool)
     for f in ULP_FIELDS[dof]:
         for direction in (np.inf, -np.inf):
-            o2 = fl["ovr"].copy()
+            o2 = ovr.copy()
             o2[f] = np.nextafter(o2[f], direction)
             fin2, _ = run(o2)
             nf2 = ~np.isfinite(fin2).all(axis=-1)
@@ -251,6 +250,15 @@ def mixed_reference(O, m, every=500):
     R = Reference(fin, None, spread, nf_flip)
     R.den = den
     return R
+
+
+def mixed_reference(O, m, every=500):
+    """final_reference for one model of the mixed fleet, to each vehicle's horizon."""
+    dof, n, flat, fl = m["dof"], m["n"], m["flat"], m["fl"]
+    ctrl = ("knots", fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
+    T = int(m["horizon"].max())
+    init_fn = (lambda o: O.init24(flat, m["recs"], False, o)) if dof == 24 else (lambda o: None)
+    return final_reference(O, dof, flat, ctrl, n, T, fl["ovr"], init_fn, m["horizon"], every)
 
 
 def final_err(a, b, den):

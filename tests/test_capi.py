@@ -223,3 +223,42 @@ def test_parsing_ignores_the_process_locale():
             % (os.path.join(ROOT, "low-fidelity-dynamic-models_amd"), usable, veh, tire))
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True).stdout.strip()
     assert bytes.fromhex(out) == lfdm_amd.params_from_json(18, veh, tire).tobytes()
+
+
+def test_vehicle_params_get_set_without_gpu():
+    """lfdm_get_vehicle_params returns a vehicle's set with its per-vehicle parameters applied;
+    lfdm_set_vehicle_params gives one vehicle any parameter vector (host side, before the device image:
+    no GPU is touched)."""
+    d = f"{DATA}/18dof"
+    hm = lfdm_amd.params_from_json(18, f"{d}/HMMWV/vehicle.json", f"{d}/HMMWV/tmeasy.json")
+    s = SolverHalfImplicitGPU(18, 10)
+    s.construct_flat(hm, 6, None, TMeasy)
+    s.construct_flat(hm, 4, None, TMeasy)
+    np.testing.assert_array_equal(s.get_vehicle_params(7), hm)
+    from lfdm_amd import workloads as W
+
+    ovr = W.default_overrides(hm, 10)
+    ovr[0, 3] *= 1.1   # mass
+    ovr[5, 3] = 0.5    # torque scale
+    ovr[6, 3] = 777.0  # brake torque
+    s.set_overrides(ovr)
+    got = s.get_vehicle_params(3)
+    F = W.FLAT18
+    assert got[F["m"]] == hm[F["m"]] * 1.1 and got[F["maxBrakeTorque"]] == 777.0
+    changed = np.flatnonzero(got != hm)
+    assert set(changed) - {F["m"], F["maxBrakeTorque"]}  # the torque map ordinates, halved
+    np.testing.assert_array_equal(s.get_vehicle_params(4), hm)
+    mod = hm.copy()
+    mod[F["h"]] += 0.1
+    s.set_vehicle_params(2, mod)
+    np.testing.assert_array_equal(s.get_vehicle_params(2), mod)  # its per-vehicle parameters follow the vector
+    np.testing.assert_array_equal(s.get_vehicle_params(1), hm)
+    s.set_vehicle_params(8, mod)  # joins vehicle 2's set (same vector, same driver table)
+    np.testing.assert_array_equal(s.get_vehicle_params(8), mod)
+    with pytest.raises(LfdmError):
+        s.set_vehicle_params(10, mod)
+    bad = mod.copy()
+    bad[F["gears"]] = 0
+    with pytest.raises(LfdmError):
+        s.set_vehicle_params(0, bad)
+    s.close()

@@ -217,6 +217,73 @@ def test_batch_independence(golden18):
     s3.close()
 
 
+def test_per_vehicle_parameters_vs_oracle(golden18):
+    """Vehicles given parameters of their own through set_vehicle_params, fields beyond the seven
+    per-vehicle ones (CG height, track width, a shared wheelbase change, tire stiffness, steering limit),
+    before the first launch and again after 5 s (which lays the slots out anew with the records kept):
+    every vehicle within 1e-9 of the oracle run with its own parameter vector over 10 s; vehicles that
+    were never edited are bit-identical to the same fleet without edits."""
+    import lfdm_oracle as O
+
+    say("[gpu18] per-vehicle parameter sets")
+    flat = golden18["hmmwv_acc3/params"]
+    n, half = 256, 5000
+    fl = W.knot_fleet(n, flat, seed=12)
+    F = W.FLAT18
+    cx = W.tire_offsets(flat)[0] - 25 + 6
+
+    def edited(idx, k, fac):
+        f = flat.copy()
+        f[idx] *= fac
+        return k, f
+
+    first = dict([edited(F["h"], 5, 1.15), edited(7, 17, 1.1), edited(cx, 100, 0.8), edited(F["maxSteer"], 200, 0.9)] +
+                 [edited(F["a"], v, 1.03) for v in range(40, 48)])
+    later = dict([edited(F["b"], 17, 0.97), edited(F["h"], 60, 0.9)])
+
+    def solver():
+        s = SolverHalfImplicitGPU(18, n)
+        s.construct_flat(flat, n, None, TMeasy)
+        s.Initialize(s.zero_record(), n)
+        s.set_overrides(fl["ovr"])
+        s.set_controls_knots(fl["ns"], fl["ds"], fl["np"], fl["dp"], fl["knots"])
+        return s
+
+    plain = solver()
+    plain.advance(2 * half)
+    base = plain.get_states()
+    plain.close()
+    s = solver()
+    for v, f in first.items():
+        s.set_vehicle_params(v, f)
+    s.advance(half)
+    mid = s.get_states()
+    for v, f in later.items():
+        s.set_vehicle_params(v, f)
+    np.testing.assert_array_equal(s.get_states(), mid)  # the re-layout keeps every record
+    s.advance(half)
+    gpu = s.get_states()
+    assert np.array_equal(s.get_vehicle_params(17), later[17]) and np.array_equal(s.get_vehicle_params(5), first[5])
+    s.close()
+    untouched = [v for v in range(n) if v not in first and v not in later]
+    np.testing.assert_array_equal(gpu[untouched], base[untouched])
+    worst = 0.0
+    for v in sorted(set(first) | set(later)):
+        f0 = first.get(v, flat)
+        f1 = later.get(v, f0)
+        k = fl["knots"][:, v:v + 1]
+        ctrl = ("knots", fl["ns"], fl["ds"], fl["np"], fl["dp"], k)
+        o0 = fl["ovr"][:, v:v + 1] if v not in first else W.default_overrides(f0, 1)
+        o1 = o0 if v not in later else W.default_overrides(f1, 1)
+        m1, _ = O.run(18, f0, ctrl, 1, half, ovr=o0)
+        # the second half starts at the time the solver reached by accumulating t += dt, as the oracle's loop
+        m2, _ = O.run(18, f1, ctrl, 1, half, ovr=o1, init=m1, t0=W.time_grid(half + 1, 1e-3)[half])
+        den = np.maximum(np.abs(m2[0]), 1.0)
+        worst = max(worst, float((np.abs(gpu[v] - m2[0]) / den).max()))
+    say(f"[gpu18] per-vehicle parameter sets: worst relative difference to the oracle {worst:.2e}")
+    assert worst <= 1e-9
+
+
 def test_chunking_and_block_size_invariance(golden18):
     """Results do not depend on kernel_sim_time (launch chunking) or threads per block."""
     say("[gpu18] chunking invariance")
