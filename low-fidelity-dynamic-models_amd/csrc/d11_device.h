@@ -53,14 +53,14 @@ LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, doubl
                                        double throttle, double braking, double dt, X11* x) {
     const TireP& t = P.tire;
     // computeTireLoads
-    const double Z2 = qdiv(k.Z4n * (s.udot - s.wz * s.v), k.two_ab);
+    const double Z2 = xdiv<STRICT>(k.Z4n * (s.udot - s.wz * s.v), k.two_ab);
     double fz[2];
     fz[0] = k.Z1 - Z2;
     fz[1] = k.Z3 + Z2;
     fz[0] = fz[0] > 0. ? fz[0] : 0.;
     fz[1] = fz[1] > 0. ? fz[1] : 0.;
     // vehToTireTransform (dof11.cpp:45-71)
-    const double delta = P.nonLinearSteer ? map_y(P.steer, steering) : steering * P.maxSteer;
+    const double delta = P.nonLinearSteer ? map_y<STRICT>(P.steer, steering) : steering * P.maxSteer;
     double sd, cd;
     sincos(delta, &sd, &cd);
     double vsx[2], vsy[2];
@@ -73,7 +73,7 @@ LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, doubl
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         const double dl = i == 0 ? delta : u.delta_rear;
-        const double d = i == 0 ? qdiv(sd, cd) : u.tan_delta_rear;
+        const double d = i == 0 ? xdiv<STRICT>(sd, cd) : u.tan_delta_rear;
         if (fz[i] <= 0) {
             if (!NR) { s.xe[i] = 0; s.ye[i] = 0; }
             s.omega[i] = 0; xt[i] = 0; rStat[i] = t.r0; fx[i] = 0; fy[i] = 0; My[i] = 0;
@@ -94,31 +94,31 @@ LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, doubl
     const double gr = pick(P.gears, P.ngears, s.gear);
     double shift_w;
     if (P.tcbool) {
-        const double omega_out = qdiv(0.5 * (s.omega[0] + s.omega[1]), gr);
+        const double omega_out = xdiv<STRICT>(0.5 * (s.omega[0] + s.omega[1]), gr);
         const double omega_in = s.crank;
         double sr = 0;
         bool rev = false;
         if (!(omega_out < 1e-9 || omega_in < 1e-9)) {
-            sr = qdiv(omega_out, omega_in);
+            sr = xdiv<STRICT>(omega_out, omega_in);
             if (sr > 1.) {
                 sr = 1. - (sr - 1.);
                 rev = true;
             }
             if (sr < 0) sr = 0;
         }
-        const double cf = map_y(P.cfmap, sr);
-        const double tr = map_y(P.trmap, sr);
-        const double q = qdiv(omega_in, cf);
+        const double cf = map_y<STRICT>(P.cfmap, sr);
+        const double tr = map_y<STRICT>(P.trmap, sr);
+        const double q = xdiv<STRICT>(omega_in, cf);
         double torque_in = -(q * q);
         if (rev) torque_in = -torque_in;
         const double torque_out = rev ? -torque_in : -tr * torque_in;
-        torque_t = qdiv(torque_out, gr);
+        torque_t = xdiv<STRICT>(torque_out, gr);
         if (s.u < 1e-9 && torque_t < 0) torque_t = 0;
-        dcrank = u.inv_ci * (drive_torque(P, k.tqs, throttle, s.crank) + torque_in);
+        dcrank = u.inv_ci * (drive_torque<STRICT>(P, k.tqs, throttle, s.crank) + torque_in);
         shift_w = omega_out;
     } else {
-        s.crank = qdiv(0.5 * (s.omega[0] + s.omega[1]), gr);
-        torque_t = qdiv(drive_torque(P, k.tqs, throttle, s.crank), gr);
+        s.crank = xdiv<STRICT>(0.5 * (s.omega[0] + s.omega[1]), gr);
+        torque_t = xdiv<STRICT>(drive_torque<STRICT>(P, k.tqs, throttle, s.crank), gr);
         if (s.u < 1e-9 && torque_t < 0) torque_t = 0;
         shift_w = s.crank;
     }
@@ -138,7 +138,7 @@ LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, doubl
     // (rear axle in the "left" slot), else whichWheels picks the driven axle
     double eng[2];
     if (P.driveType) {
-        diff_split(torque_t, s.omega[1], s.omega[0], eng[1], eng[0]);
+        diff_split<STRICT>(torque_t, s.omega[1], s.omega[0], eng[1], eng[0]);
     } else if (P.whichWheels) {
         eng[1] = torque_t;
         eng[0] = 0;
@@ -159,9 +159,9 @@ LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, doubl
         fy[0] = gy;
     }
     // computeVehRHS (dof11.cpp:893-905)
-    const double vdot = -s.u * s.wz + qdiv(fy[0] + fy[1], k.mt);
-    const double udot = s.v * s.wz + qdiv(fx[0] + fx[1], k.mt);
-    const double wzdot = qdiv(P.a * fy[0] - P.b * fy[1], k.jz);
+    const double vdot = -s.u * s.wz + xdiv<STRICT>(fy[0] + fy[1], k.mt);
+    const double udot = s.v * s.wz + xdiv<STRICT>(fx[0] + fx[1], k.mt);
+    const double wzdot = xdiv<STRICT>(P.a * fy[0] - P.b * fy[1], k.jz);
     // Half-implicit update (dof11_halfImplicit.cpp:190-226)
 #pragma unroll
     for (int i = 0; i < 2; ++i) {

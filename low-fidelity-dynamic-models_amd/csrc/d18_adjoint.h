@@ -47,7 +47,7 @@ LFDM_HD double map_slope(const Map& m, double x) {
     int r = 0;
     for (int k = 1; k < n - 1; ++k) r += (m.e[k].x < x) ? 1 : 0;
     r += 1;
-    return qdiv(m.e[r].y - m.e[r - 1].y, m.e[r].x - m.e[r - 1].x);
+    return fdiv(m.e[r].y - m.e[r - 1].y, m.e[r].x - m.e[r - 1].x);
 }
 
 // map_y(m, x) and the slope of the segment it uses (0 where it clamps), from one scan of the knots.
@@ -61,8 +61,8 @@ LFDM_HD double map_y_slope(const Map& m, double x, double& slope) {
     r += 1;
     const MapXY L = m.e[r - 1];
     const MapXY R = m.e[r];
-    slope = qdiv(R.y - L.y, R.x - L.x);
-    const double mbar = qdiv(x - L.x, R.x - L.x);
+    slope = fdiv(R.y - L.y, R.x - L.x);
+    const double mbar = fdiv(x - L.x, R.x - L.x);
     return L.y + mbar * (R.y - L.y);
 }
 
@@ -75,7 +75,7 @@ LFDM_HD double sine_step_weight_d(double x) {
 
 LFDM_HD void hypot_adj(double a, double b, double h, double hb, double& ab, double& bb) {
     if (h > 0.0) {
-        const double r = qdiv(hb, h);
+        const double r = fdiv(hb, h);
         ab += r * a;
         bb += r * b;
     }
@@ -91,7 +91,7 @@ LFDM_HD void tmxy_adj(double s, double df0, double sm, double fm, double ss, dou
     double df0loc = 0.0;
     int which = 0;  // 1: df0loc = df0, 2: df0loc = 2 fm / sm
     if (sm > 0.0) {
-        const double c = qdiv(2.0 * fm, sm);
+        const double c = fdiv(2.0 * fm, sm);
         if (c < df0) {
             df0loc = df0;
             which = 1;
@@ -101,17 +101,17 @@ LFDM_HD void tmxy_adj(double s, double df0, double sm, double fm, double ss, dou
         }
     }
     if (!(s > 0.0 && df0loc > 0.0)) return;
-    const double rs = qdiv(1.0, s), rsm = qdiv(1.0, sm), rfm = qdiv(1.0, fm);
+    const double rs = frcp(s), rsm = frcp(sm), rfm = frcp(fm);
     double dlb = 0.0;
     if (s > ss) {
         fsb += fb + fosb * rs;
         sb -= fosb * fs * rs * rs;
     } else if (s < sm) {
-        const double p = qdiv(df0loc * sm, fm) - 2.0;
-        const double sn = qdiv(s, sm);
+        const double p = fdiv(df0loc * sm, fm) - 2.0;
+        const double sn = fdiv(s, sm);
         const double dn = 1.0 + (sn + p) * sn;
-        const double f = qdiv(df0loc * sm * sn, dn);
-        const double rdn = qdiv(1.0, dn);
+        const double f = fdiv(df0loc * sm * sn, dn);
+        const double rdn = frcp(dn);
         dlb += fosb * rdn;
         double dnb = -fosb * df0loc * rdn * rdn;
         dlb += fb * sm * sn * rdn;
@@ -126,10 +126,10 @@ LFDM_HD void tmxy_adj(double s, double df0, double sm, double fm, double ss, dou
         smb += pb * df0loc * rfm;
         fmb -= pb * df0loc * sm * rfm * rfm;
     } else {
-        const double q = qdiv(fm, sm);
-        const double a = qdiv(q * q, df0loc * sm);
+        const double q = fdiv(fm, sm);
+        const double a = fdiv(q * q, df0loc * sm);
         const double h = a * (ss - sm);
-        const double sstar = sm + qdiv(fm - fs, h);
+        const double sstar = sm + fdiv(fm - fs, h);
         double f, ab = 0.0, sstarb = 0.0;
         bool use_sstar = false;
         if (sstar <= ss) {
@@ -138,11 +138,11 @@ LFDM_HD void tmxy_adj(double s, double df0, double sm, double fm, double ss, dou
                 const double d = s - sm;
                 f = fm - a * d * d;
             } else {
-                const double b = qdiv(a * (sstar - sm), ss - sstar);
+                const double b = fdiv(a * (sstar - sm), ss - sstar);
                 f = fs + b * (ss - s) * (ss - s);
             }
         } else {
-            const double sn = qdiv(s - sm, ss - sm);
+            const double sn = fdiv(s - sm, ss - sm);
             f = fm - (fm - fs) * sn * sn * (3.0 - 2.0 * sn);
         }
         const double ftb = fb + fosb * rs;
@@ -156,7 +156,7 @@ LFDM_HD void tmxy_adj(double s, double df0, double sm, double fm, double ss, dou
                 smb += ftb * 2.0 * a * d;
             } else {
                 const double g = ss - sstar;
-                const double rg = qdiv(1.0, g);
+                const double rg = frcp(g);
                 const double b = a * (sstar - sm) * rg;
                 const double e = ss - s;
                 fsb += ftb;
@@ -169,7 +169,7 @@ LFDM_HD void tmxy_adj(double s, double df0, double sm, double fm, double ss, dou
                 ssb -= bb * a * (sstar - sm) * rg * rg;
             }
             // sstar = sm + (fm - fs) / h, h = a (ss - sm)
-            const double rh = qdiv(1.0, h);
+            const double rh = frcp(h);
             smb += sstarb;
             fmb += sstarb * rh;
             fsb -= sstarb * rh;
@@ -179,7 +179,7 @@ LFDM_HD void tmxy_adj(double s, double df0, double sm, double fm, double ss, dou
             smb -= hb * a;
         } else {
             const double w = ss - sm;
-            const double rw = qdiv(1.0, w);
+            const double rw = frcp(w);
             const double sn = (s - sm) * rw;
             const double g = sn * sn * (3.0 - 2.0 * sn);
             fmb += ftb * (1.0 - g);
@@ -192,7 +192,7 @@ LFDM_HD void tmxy_adj(double s, double df0, double sm, double fm, double ss, dou
         // a = fm^2 / (sm^3 df0loc)
         fmb += ab * 2.0 * a * rfm;
         smb -= ab * 3.0 * a * rsm;
-        dlb -= qdiv(ab * a, df0loc);
+        dlb -= fdiv(ab * a, df0loc);
     }
     if (which == 1) {
         df0b += dlb;
@@ -230,7 +230,7 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     const double wxw = w.wxw, wyw = w.wyw;
 
     // ---- reverse ----
-    const double rhsxn = w.rhsxn, rhsyn = w.rhsyn, rvta = qdiv(1.0, vta);
+    const double rhsxn = w.rhsxn, rhsyn = w.rhsyn, rvta = frcp(vta);
     double vsx2b = 0, vyb = 0, xedotb = B.xedot, yedotb = B.yedot, xeb = 0, yeb = 0;
     // fx = wxw fxstr + (1 - wxw) fxdyn
     const double wxwb = B.fx * (fxstr - fxdyn), fxstrb = B.fx * wxw, fxdynb = B.fx * (1. - wxw);
@@ -244,8 +244,8 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     yeb += fyrb * t.cy + fydynb * t.cy;
     yedotb += fyrb * t.dy + fydynb * t.dy;
     // xedot = numx / denx, yedot = numy / deny
-    const double numxb = qdiv(xedotb, denx), denxb = -numxb * xedot;
-    const double numyb = qdiv(yedotb, deny), denyb = -numyb * yedot;
+    const double numxb = fdiv(xedotb, denx), denxb = -numxb * xedot;
+    const double numyb = fdiv(yedotb, deny), denyb = -numyb * yedot;
     double vtxsb = -numxb * t.cx * xe + denxb * t.dx;
     xeb += -numxb * vtxs * t.cx;
     double fosb = -numxb * vsx2 + denxb;
@@ -294,7 +294,7 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     double sxnb = 0, synb = 0;
     if (sc > 0) {
         // ca = sxn / sc, sa = syn / sc, sc = hypot(sxn, syn)
-        const double rsc = qdiv(1.0, sc);
+        const double rsc = frcp(sc);
         scb -= (cab * ca + sab * sa) * rsc;
         sxnb += (cab + scb * sxn) * rsc;
         synb += (sab + scb * syn) * rsc;
@@ -317,7 +317,7 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     sxmb += ssumb;
     symb += ssumb;
     // rx = fxm / dfx0, ry = fym / dfy0
-    const double rxbr = qdiv(rxb, c.dfx0), rybr = qdiv(ryb, c.dfy0);
+    const double rxbr = fdiv(rxb, c.dfx0), rybr = fdiv(ryb, c.dfy0);
     fxmb += rxbr;
     dfx0b -= rxbr * rx;
     fymb += rybr;
@@ -333,7 +333,7 @@ LFDM_HD void tire_tmeasy_adj(const TireP& t, const Derived& u, double qy1, doubl
     // sy = -tan(alpha), alpha = atan2(vsy, vta) - delta (evaluated as a tangent difference)
     const double alphab = -syb * (1.0 + ta * ta);
     B.delta -= alphab;
-    const double ar2 = qdiv(alphab, vsy * vsy + vta * vta);
+    const double ar2 = fdiv(alphab, vsy * vsy + vta * vta);
     B.vsy += ar2 * vta;
     vtab -= ar2 * vsy;
     // sx = -vsx2 / vta
@@ -371,7 +371,7 @@ LFDM_HD void diff_split_adj(double torque, double sl, double sr, double tlb, dou
         bias = 4 * (max_bias - 1) * diff + (2 - max_bias);
         dbias = 4 * (max_bias - 1);
     }
-    const double alpha = qdiv(bias, 1 + bias);
+    const double alpha = fdiv(bias, 1 + bias);
     const bool left_slow = fabs(sl) < fabs(sr);
     const double slowb = left_slow ? tlb : trb;
     const double fastb = left_slow ? trb : tlb;
@@ -380,7 +380,7 @@ LFDM_HD void diff_split_adj(double torque, double sl, double sr, double tlb, dou
     const double sb = slowb - fastb;
     torqueb += sb * alpha;
     const double alphab = sb * torque;
-    const double biasb = qdiv(alphab, (1 + bias) * (1 + bias));
+    const double biasb = fdiv(alphab, (1 + bias) * (1 + bias));
     const double diffb = biasb * dbias;
     const double sd = sgn(sl - sr);
     slb += diffb * sd;
@@ -404,13 +404,13 @@ LFDM_HD void powertrain4_adj(const PSet& P, const Derived& u, double tqs, double
     // ---- forward recompute ----
     double torque_t, torque_in = 0, torque_out = 0, sr0 = 0, q = 0, cf = 1, tr = 0, cfs = 0, trs = 0;
     bool active = false, rev = false, clip = false;
-    const double omega_out = qdiv(0.25 * osum, gr);
+    const double omega_out = fdiv(0.25 * osum, gr);
     double w_dt;  // crank speed the drive torque is evaluated at
     if (P.tcbool) {
         const double omega_in = crank;
         double sr = 0;
         if (!(omega_out < 1e-9 || omega_in < 1e-9)) {
-            sr0 = qdiv(omega_out, omega_in);
+            sr0 = fdiv(omega_out, omega_in);
             sr = sr0;
             if (sr > 1.) {
                 sr = 1. - (sr - 1.);
@@ -424,15 +424,15 @@ LFDM_HD void powertrain4_adj(const PSet& P, const Derived& u, double tqs, double
         }
         cf = map_y_slope(P.cfmap, sr, cfs);
         tr = map_y_slope(P.trmap, sr, trs);
-        q = qdiv(omega_in, cf);
+        q = fdiv(omega_in, cf);
         torque_in = -(q * q);
         if (rev) torque_in = -torque_in;
         torque_out = rev ? -torque_in : -tr * torque_in;
-        torque_t = qdiv(torque_out, gr);
+        torque_t = fdiv(torque_out, gr);
         w_dt = crank;
     } else {
-        w_dt = qdiv(0.25 * osum, gr);
-        torque_t = qdiv(drive_torque(P, tqs, throttle, w_dt), gr);
+        w_dt = fdiv(0.25 * osum, gr);
+        torque_t = fdiv(drive_torque<false>(P, tqs, throttle, w_dt), gr);
     }
     if (vel_u < 1e-9 && torque_t < 0) {
         torque_t = 0;
@@ -477,7 +477,7 @@ LFDM_HD void powertrain4_adj(const PSet& P, const Derived& u, double tqs, double
     // drive torque at w_dt: tqs * T(w) * throttle + L(w)
     double Ts;
     const double Tw = map_y_slope(P.torque, w_dt, Ts), Ls = map_slope(P.losses, w_dt);
-    const double rgr = qdiv(1.0, gr);
+    const double rgr = frcp(gr);
     if (P.tcbool) {
         const double omega_in = crank;
         double toutb = ttb * rgr;
@@ -491,13 +491,13 @@ LFDM_HD void powertrain4_adj(const PSet& P, const Derived& u, double tqs, double
         B.crank += DTb * (tqs * throttle * Ts + Ls);
         // torque_in = -q^2 (negated when reversed)
         const double qb = tinb * (rev ? 2.0 * q : -2.0 * q);
-        const double qbr = qdiv(qb, cf);
+        const double qbr = fdiv(qb, cf);
         double omega_inb = qbr;
         const double cfb = -qbr * q;
         const double srb = cfb * cfs + trb * trs;
         if (active) {
             const double sr0b = rev ? -srb : srb;
-            const double omega_outb = qdiv(sr0b, omega_in);
+            const double omega_outb = fdiv(sr0b, omega_in);
             omega_inb -= omega_outb * sr0;
             const double ob = omega_outb * 0.25 * rgr;
 #pragma unroll
@@ -573,14 +573,14 @@ LFDM_HD void step18_adj(const PSet& P, const Derived& u, const K18& k, const S18
             yedot[i] = to->yedot[i];
         } else {
             double xt;
-            tire_tmeasy(t, u, k, 0.0, i < 2 ? qdiv(sd, cd) : u.tan_delta_rear, fz[i], vsx[i], vsy[i], xe[i], ye[i],
+            tire_tmeasy(t, u, k, 0.0, i < 2 ? fdiv(sd, cd) : u.tan_delta_rear, fz[i], vsx[i], vsy[i], xe[i], ye[i],
                         xedot[i], yedot[i], om[i], fx[i], fy[i], My[i], xt, rStat[i]);
         }
     }
     double crank = s0.crank;
     int gear = s0.gear;
     double dOm[4], eng[4], dcrank;
-    powertrain4(P, u, k.tqs, k.maxBrake, throttle, braking, s0.u, crank, gear, om, fx, rStat, My, dOm, eng, dcrank);
+    powertrain4<false>(P, u, k.tqs, k.maxBrake, throttle, braking, s0.u, crank, gear, om, fx, rStat, My, dOm, eng, dcrank);
     double gfx[4], gfy[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -759,7 +759,7 @@ LFDM_HD void step18_adj(const PSet& P, const Derived& u, const K18& k, const S18
         tb.My = pb.My[i];
         tb.rStat = pb.rStat[i];
         tb.fz = tb.vsx = tb.vsy = tb.delta = tb.xe = tb.ye = tb.omega = tb.qy1 = tb.qy2 = 0.0;
-        tire_tmeasy_adj(t, u, k.qy1, k.qy2, i < 2 ? qdiv(sd, cd) : u.tan_delta_rear, fz[i], vsx[i], vsy[i], s0.xe[i],
+        tire_tmeasy_adj(t, u, k.qy1, k.qy2, i < 2 ? fdiv(sd, cd) : u.tan_delta_rear, fz[i], vsx[i], vsy[i], s0.xe[i],
                         s0.ye[i], s0.omega[i], tb);
         sb.xe[i] += xeb[i] + tb.xe;
         sb.ye[i] += yeb[i] + tb.ye;

@@ -128,7 +128,7 @@ LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, doubl
 #pragma unroll
     for (int i = 0; i < 4; ++i) fz[i] = fz[i] > 0. ? fz[i] : 0.;
     // vehToTireTransform
-    const double delta = P.nonLinearSteer ? map_y(P.steer, steering) : steering * P.maxSteer;
+    const double delta = P.nonLinearSteer ? map_y<STRICT>(P.steer, steering) : steering * P.maxSteer;
     double sd, cd;
     sincos(delta, &sd, &cd);
     double vsx[4], vsy[4];
@@ -145,7 +145,7 @@ LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, doubl
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const double dl = i < 2 ? delta : u.delta_rear;
-        const double d = i < 2 ? qdiv(sd, cd) : u.tan_delta_rear;
+        const double d = i < 2 ? xdiv<STRICT>(sd, cd) : u.tan_delta_rear;
         if (!NR)
             tire_tmeasy<STRICT>(t, u, k, dl, d, fz[i], vsx[i], vsy[i], s.xe[i], s.ye[i], s.xedot[i], s.yedot[i],
                                 s.omega[i], fx[i], fy[i], My[i], xt[i], rStat[i]);
@@ -162,7 +162,7 @@ LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, doubl
     }
     // computePowertrainRHS
     double dOm[4], eng[4], dcrank;
-    powertrain4(P, u, k.tqs, k.maxBrake, throttle, braking, s.u, s.crank, s.gear, s.omega, fx, rStat, My, dOm, eng,
+    powertrain4<STRICT>(P, u, k.tqs, k.maxBrake, throttle, braking, s.u, s.crank, s.gear, s.omega, fx, rStat, My, dOm, eng,
                 dcrank);
     // tireToVehTransform (front tires only)
 #pragma unroll

@@ -107,7 +107,7 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
         dvu[i] = dvs[i] + (-s.dxs[i] * s.wx + ls[i] * s.wxdot);
     }
     // vehToTireTransform (dof24.cpp:313-385) with last step's static radius
-    const double delta = P.nonLinearSteer ? map_y(P.steer, steering) : steering * P.maxSteer;
+    const double delta = P.nonLinearSteer ? map_y<STRICT>(P.steer, steering) : steering * P.maxSteer;
     double sd, cd;
     sincos(delta, &sd, &cd);
     double vgx[4], vgy[4], vsx[4], vsy[4];
@@ -121,11 +121,11 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     // computeTireRHS (dof24.cpp:827-1121): load from the compression, radius from the attitude
     double fx[4], fy[4], fz[4], My[4];
     const double cc = cth * cph;
-    const double rcc = qdiv(1., cc);
+    const double rcc = frcp(cc);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const double dl = i < 2 ? delta : u.delta_rear;
-        const double d = i < 2 ? qdiv(sd, cd) : u.tan_delta_rear;
+        const double d = i < 2 ? xdiv<STRICT>(sd, cd) : u.tan_delta_rear;
         fz[i] = s.xt[i] * t.kt;
         s.rStat[i] = STRICT ? qdiv(t.r0 - s.xt[i], cos(s.theta) * cos(s.phi)) : (t.r0 - s.xt[i]) * rcc;
         if (fz[i] <= 0) {
@@ -146,7 +146,7 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     for (int i = 0; i < 4; ++i) dxt[i] = s.vsz[i] - (cth * (s.wu[i] * cph + vu[i] * sph) - uu[i] * sth);
     // computePowertrainRHS (dof24.cpp:1338-1509)
     double dOm[4], eng[4], dcrank;
-    powertrain4(P, u, k.tqs, k.maxBrake, throttle, braking, s.u, s.crank, s.gear, s.omega, fx, s.rStat, My, dOm, eng,
+    powertrain4<STRICT>(P, u, k.tqs, k.maxBrake, throttle, braking, s.u, s.crank, s.gear, s.omega, fx, s.rStat, My, dOm, eng,
                 dcrank);
     // tireToVehTransform (dof24.cpp:459-528)
 #pragma unroll
@@ -240,7 +240,7 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     s.wz = s.wz + wzdot * dt;
     // Euler-angle rates at the step's (old) theta and phi: tan(theta) and 1/cos(theta) from the
     // step's sine and cosine (same values, other rounding than tan() and two divisions)
-    const double rcth = qdiv(1., cth);
+    const double rcth = frcp(cth);
     const double tth = STRICT ? tan(s.theta) : sth * rcth;
     const double dtheta = s.wy * cph - s.wz * sph;
     const double cth_e = STRICT ? cos(s.theta) : cth;

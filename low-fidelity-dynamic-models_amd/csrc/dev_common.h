@@ -72,6 +72,62 @@ LFDM_HD double qsqrt(double x) {
 #endif
 }
 
+// Quotient, reciprocal and square root of the fast arithmetic form: the same refinements stopped one
+// step earlier.  fdiv: one Newton-Raphson iteration on v_rcp_f64, then the quotient with one
+// remainder correction (6 VALU instructions); frcp: two iterations (5); fsqrt: one coupled iteration
+// from v_rsq_f64 and one remainder correction (9; the operand of v_rsq_f64 is raised to 2^-1000 so
+// that x = 0 runs through to 0 without a select).  Each result is within one ulp of the correctly
+// rounded one, which the refinement's error bound (the square of the first iteration's, itself the
+// square of the hardware estimate's) makes the correctly rounded value for all but operands within a
+// minute fraction of an ulp of a rounding boundary.  Same operand ranges as qdiv and qsqrt.  Host
+// builds use `/` and sqrt.
+LFDM_HD double fdiv(double n, double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(d);
+    const double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    const double q = n * r;
+    const double rem = __builtin_fma(-d, q, n);
+    return __builtin_fma(rem, r, q);
+#else
+    return n / d;
+#endif
+}
+
+LFDM_HD double frcp(double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    return __builtin_fma(r, e, r);
+#else
+    return 1.0 / d;
+#endif
+}
+
+LFDM_HD double fsqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(__builtin_fmax(x, 0x1p-1000));
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    const double d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+#else
+    return sqrt(x);
+#endif
+}
+
+// The quotient of the arithmetic form: correctly rounded (the reference's `/`) in the strict form,
+// faithful in the fast one.
+template <bool STRICT>
+LFDM_HD double xdiv(double n, double d) {
+    return STRICT ? qdiv(n, d) : fdiv(n, d);
+}
+
 LFDM_HD double sgn(double v) { return (double)((0.0 < v) - (v < 0.0)); }
 
 LFDM_HD double clampd(double v, double lo, double hi) {
@@ -87,16 +143,18 @@ LFDM_HD double sine_step_weight(double x) {
 }
 
 // General sineStep (utils.cpp:128-138).
+template <bool STRICT>
 LFDM_HD double sine_step(double x, double x1, double y1, double x2, double y2) {
     if (x <= x1) return y1;
     if (x >= x2) return y2;
     const double dx = x2 - x1, dy = y2 - y1;
-    return y1 + qdiv(dy * (x - x1), dx) - qdiv(dy, k2Pi) * sin(qdiv(k2Pi * (x - x1), dx));
+    return y1 + xdiv<STRICT>(dy * (x - x1), dx) - xdiv<STRICT>(dy, k2Pi) * sin(xdiv<STRICT>(k2Pi * (x - x1), dx));
 }
 
 // Clamped piecewise-linear lookup (utils.cpp:109-125). `m` is wave-uniform, so the knot scan
 // runs on scalar loads; the interval index counts knots strictly below x, which is exactly
 // std::lower_bound's answer on a sorted map.
+template <bool STRICT>
 LFDM_HD double map_y(const Map& m, double x) {
     const int n = m.n;
     if (x <= m.e[0].x) return m.e[0].y;
@@ -106,12 +164,13 @@ LFDM_HD double map_y(const Map& m, double x) {
     r += 1;
     const MapXY L = m.e[r - 1];
     const MapXY R = m.e[r];
-    const double mbar = qdiv(x - L.x, R.x - L.x);
+    const double mbar = xdiv<STRICT>(x - L.x, R.x - L.x);
     return L.y + mbar * (R.y - L.y);
 }
 
 // Same lookup on a map whose abscissae are scaled by sx and ordinates by sy1 then sy2
 // (multiplying by 1.0 is exact, so unit factors reproduce the unscaled map bit for bit).
+template <bool STRICT>
 LFDM_HD double map_y_scaled(const Map& m, double x, double sx, double sy1, double sy2) {
     const int n = m.n;
     if (x <= m.e[0].x * sx) return m.e[0].y * sy1 * sy2;
@@ -122,7 +181,7 @@ LFDM_HD double map_y_scaled(const Map& m, double x, double sx, double sy1, doubl
     const MapXY L = m.e[r - 1];
     const MapXY R = m.e[r];
     const double lx = L.x * sx, rx = R.x * sx, ly = L.y * sy1 * sy2, ry = R.y * sy1 * sy2;
-    const double mbar = qdiv(x - lx, rx - lx);
+    const double mbar = xdiv<STRICT>(x - lx, rx - lx);
     return ly + mbar * (ry - ly);
 }
 

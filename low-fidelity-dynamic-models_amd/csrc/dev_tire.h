@@ -16,42 +16,42 @@ LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, d
                                            double& f) {
     double df0loc = 0.0;
     if (sm > 0.0) {
-        const double c = qdiv(2.0 * fm, sm);
+        const double c = xdiv<STRICT>(2.0 * fm, sm);
         df0loc = (c < df0) ? df0 : c;
     }
     double fos;
     if (s > 0.0 && df0loc > 0.0) {
         if (s > ss) {
             f = fs;
-            fos = qdiv(f, s);
+            fos = xdiv<STRICT>(f, s);
         } else if (s < sm) {
-            const double p = qdiv(df0loc * sm, fm) - 2.0;
-            const double sn = qdiv(s, sm);
+            const double p = xdiv<STRICT>(df0loc * sm, fm) - 2.0;
+            const double sn = xdiv<STRICT>(s, sm);
             const double dn = 1.0 + (sn + p) * sn;
             if (STRICT) {
-                f = qdiv(df0loc * sm * sn, dn);
-                fos = qdiv(df0loc, dn);
+                f = xdiv<STRICT>(df0loc * sm * sn, dn);
+                fos = xdiv<STRICT>(df0loc, dn);
             } else {
-                const double rdn = qdiv(1.0, dn);
+                const double rdn = frcp(dn);
                 f = df0loc * sm * sn * rdn;
                 fos = df0loc * rdn;
             }
         } else {
-            const double q = qdiv(fm, sm);
-            const double a = qdiv(q * q, df0loc * sm);
-            const double sstar = sm + qdiv(fm - fs, a * (ss - sm));
+            const double q = xdiv<STRICT>(fm, sm);
+            const double a = xdiv<STRICT>(q * q, df0loc * sm);
+            const double sstar = sm + xdiv<STRICT>(fm - fs, a * (ss - sm));
             if (sstar <= ss) {
                 if (s <= sstar) {
                     f = fm - a * (s - sm) * (s - sm);
                 } else {
-                    const double b = qdiv(a * (sstar - sm), ss - sstar);
+                    const double b = xdiv<STRICT>(a * (sstar - sm), ss - sstar);
                     f = fs + b * (ss - s) * (ss - s);
                 }
             } else {
-                const double sn = qdiv(s - sm, ss - sm);
+                const double sn = xdiv<STRICT>(s - sm, ss - sm);
                 f = fm - (fm - fs) * sn * sn * (3.0 - 2.0 * sn);
             }
-            fos = qdiv(f, s);
+            fos = xdiv<STRICT>(f, s);
         }
     } else {
         f = 0.0;
@@ -101,7 +101,7 @@ LFDM_HD void curve_at(const TireP& t, const Derived& u, double qy1, double qy2, 
 // steering angle, the device libm's hypot, and every quotient as its own (correctly rounded)
 // division, so that they differ from the reference only where the device libm and glibc round
 // differently.
-LFDM_HD double hyp(double a, double b) { return qsqrt(a * a + b * b); }
+LFDM_HD double hyp(double a, double b) { return fsqrt(a * a + b * b); }
 
 template <bool STRICT>
 LFDM_HD double hyp2(double a, double b) {
@@ -147,18 +147,18 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
         w.sx = qdiv(-w.vsx2, w.vta);
         w.sy = -tan(atan2(vsy, w.vta) - delta);
     } else {
-        const double rvta = qdiv(1., w.vta);
+        const double rvta = frcp(w.vta);
         w.sx = -w.vsx2 * rvta;
         const double r = vsy * rvta;  // tan(atan2(vsy, vta))
-        w.sy = -qdiv(r - tan_delta, 1. + r * tan_delta);
+        w.sy = -fdiv(r - tan_delta, 1. + r * tan_delta);
     }
     w.clip = fz > t.pnmax;
     w.fzc = w.clip ? t.pnmax : fz;
     const Curve& c = w.c;
     curve_at<STRICT>(t, u, qy1, qy2, w.fzc, w.c);
     w.fzn = w.fzc * u.inv_pn;
-    w.rx = qdiv(c.fxm, c.dfx0);
-    w.ry = qdiv(c.fym, c.dfy0);
+    w.rx = xdiv<STRICT>(c.fxm, c.dfx0);
+    w.ry = xdiv<STRICT>(c.fym, c.dfy0);
     w.ssum = c.sxm + c.sym;
     w.rsum = w.rx + w.ry;
     if (STRICT) {
@@ -175,17 +175,17 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
             w.sa = kHalfSqrt2;
         }
     } else {
-        w.rssum = qdiv(1., w.ssum);
-        w.rrsum = qdiv(1., w.rsum);
+        w.rssum = frcp(w.ssum);
+        w.rrsum = frcp(w.rsum);
         w.hsxn = c.sxm * w.rssum + w.rx * w.rrsum;
         w.hsyn = c.sym * w.rssum + w.ry * w.rrsum;
-        w.rhsxn = qdiv(1., w.hsxn);
-        w.rhsyn = qdiv(1., w.hsyn);
+        w.rhsxn = frcp(w.hsxn);
+        w.rhsyn = frcp(w.hsyn);
         w.sxn = w.sx * w.rhsxn;
         w.syn = w.sy * w.rhsyn;
         w.sc = hyp(w.sxn, w.syn);
         if (w.sc > 0) {
-            const double rsc = qdiv(1., w.sc);
+            const double rsc = frcp(w.sc);
             w.ca = w.sxn * rsc;
             w.sa = w.syn * rsc;
         } else {
@@ -230,8 +230,8 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
     w.denx = w.vtxs * t.dx + w.fos;
     w.numy = -w.vtys * t.cy * ye - w.fos * w.vy;
     w.deny = w.vtys * t.dy + w.fos;
-    w.xedot = qdiv(w.numx, w.denx);
-    w.yedot = qdiv(w.numy, w.deny);
+    w.xedot = xdiv<STRICT>(w.numx, w.denx);
+    w.yedot = xdiv<STRICT>(w.numy, w.deny);
     // the reference forms dx * num / den as (dx * num) / den here, but dy * (num / den) below
     w.fxdyn = (STRICT ? qdiv(t.dx * w.numx, w.denx) : t.dx * w.xedot) + t.cx * xe;
     w.fydyn = t.dy * w.yedot + (t.cy * ye);
@@ -276,11 +276,11 @@ LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double
     const double r_eff = rdynco * t.r0 + (1. - rdynco) * rStat;
     vsx = vsx - (omega * r_eff);
     const double vta = r_eff * fabs(omega) + 0.01;
-    double Fx0 = tanh(qdiv(-2.0 * vsx, t.vcoulomb)) * fz * t.mu;
-    double Fy0 = tanh(qdiv(-2.0 * vsy, t.vcoulomb)) * fz * t.mu;
+    double Fx0 = tanh(xdiv<STRICT>(-2.0 * vsx, t.vcoulomb)) * fz * t.mu;
+    double Fy0 = tanh(xdiv<STRICT>(-2.0 * vsy, t.vcoulomb)) * fz * t.mu;
     const double F0 = hyp2<STRICT>(Fx0, Fy0);
     if (F0 > fz * t.mu) {
-        const double fsc = qdiv(fz * t.mu, F0);
+        const double fsc = xdiv<STRICT>(fz * t.mu, F0);
         Fx0 = Fx0 * fsc;
         Fy0 = Fy0 * fsc;
     }
@@ -289,18 +289,18 @@ LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double
         sx = qdiv(-vsx, vta);
         sy = -tan(atan2(vsy, vta) - delta);
     } else {
-        const double rvta = qdiv(1., vta);
+        const double rvta = frcp(vta);
         sx = -vsx * rvta;
         const double r = vsy * rvta;  // tan(atan2(vsy, vta))
-        sy = -qdiv(r - tan_delta, 1. + r * tan_delta);
+        sy = -fdiv(r - tan_delta, 1. + r * tan_delta);
     }
     if (fz > t.pnmax) fz = t.pnmax;
     Curve c;
     curve_at<STRICT>(t, u, qy1, qy2, fz, c);
-    const double frblend = sine_step(fabs(vsx_in), t.frblend_begin, 0., t.frblend_end, 1.);
+    const double frblend = sine_step<STRICT>(fabs(vsx_in), t.frblend_begin, 0., t.frblend_end, 1.);
     const double sc = hyp2<STRICT>(sx, sy);
     double ca, sa;
-    const double rsc = (!STRICT && sc > 0) ? qdiv(1., sc) : 0.;
+    const double rsc = (!STRICT && sc > 0) ? frcp(sc) : 0.;
     if (sc > 0) {
         ca = STRICT ? qdiv(sx, sc) : sx * rsc;
         sa = STRICT ? qdiv(sy, sc) : sy * rsc;
@@ -332,6 +332,7 @@ LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double
 }
 
 // differentialSplit (dof18.cpp:46-73) with max_bias = 2.
+template <bool STRICT>
 LFDM_HD void diff_split(double torque, double sl, double sr, double& tl, double& tr) {
     const double max_bias = 2;
     const double diff = fabs(sl - sr);
@@ -340,7 +341,7 @@ LFDM_HD void diff_split(double torque, double sl, double sr, double& tl, double&
         bias = max_bias;
     else if (diff > 0.25)
         bias = 4 * (max_bias - 1) * diff + (2 - max_bias);
-    const double alpha = qdiv(bias, 1 + bias);
+    const double alpha = xdiv<STRICT>(bias, 1 + bias);
     const double slow = alpha * torque;
     const double fast = torque - slow;
     const bool left_slow = fabs(sl) < fabs(sr);
@@ -350,18 +351,20 @@ LFDM_HD void diff_split(double torque, double sl, double sr, double& tl, double&
 
 // driveTorque (dof18.cpp:15-42, dof11.cpp:15-42, dof24.cpp:92-119) with the per-vehicle torque
 // scale tqs on the map ordinates.
+template <bool STRICT>
 LFDM_HD double drive_torque(const PSet& P, double tqs, double throttle, double w) {
     if (P.throttleMod) {
-        const double tq = map_y_scaled(P.torque, w, throttle, tqs, throttle);
-        return tq + map_y(P.losses, w);
+        const double tq = map_y_scaled<STRICT>(P.torque, w, throttle, tqs, throttle);
+        return tq + map_y<STRICT>(P.losses, w);
     }
-    const double tq = map_y_scaled(P.torque, w, 1.0, tqs, 1.0);
-    return tq * throttle + map_y(P.losses, w);
+    const double tq = map_y_scaled<STRICT>(P.torque, w, 1.0, tqs, 1.0);
+    return tq * throttle + map_y<STRICT>(P.losses, w);
 }
 
 // computePowertrainRHS of the four-wheel models (dof18.cpp:675-845, dof24.cpp:1338-1509): torque
 // converter or slaved crank, gear shift for the next step, axle and differential split, wheel
 // accelerations.  Updates crank (without a converter) and gear; returns dOm, eng and dcrank.
+template <bool STRICT>
 LFDM_HD void powertrain4(const PSet& P, const Derived& u, double tqs, double maxBrake,
                                             double throttle, double braking, double vel_u, double& crank, int& gear,
                                             const double* omega, const double* fx, const double* rStat,
@@ -372,31 +375,31 @@ LFDM_HD void powertrain4(const PSet& P, const Derived& u, double tqs, double max
     const double gr = pick(P.gears, P.ngears, gear);
     double shift_w;
     if (P.tcbool) {
-        const double omega_out = qdiv(0.25 * osum, gr);
+        const double omega_out = xdiv<STRICT>(0.25 * osum, gr);
         const double omega_in = crank;
         double sr = 0;
         bool rev = false;
         if (!(omega_out < 1e-9 || omega_in < 1e-9)) {
-            sr = qdiv(omega_out, omega_in);
+            sr = xdiv<STRICT>(omega_out, omega_in);
             if (sr > 1.) {
                 sr = 1. - (sr - 1.);
                 rev = true;
             }
             if (sr < 0) sr = 0;
         }
-        const double cf = map_y(P.cfmap, sr);
-        const double tr = map_y(P.trmap, sr);
-        const double q = qdiv(omega_in, cf);
+        const double cf = map_y<STRICT>(P.cfmap, sr);
+        const double tr = map_y<STRICT>(P.trmap, sr);
+        const double q = xdiv<STRICT>(omega_in, cf);
         double torque_in = -(q * q);
         if (rev) torque_in = -torque_in;
         const double torque_out = rev ? -torque_in : -tr * torque_in;
-        torque_t = qdiv(torque_out, gr);
+        torque_t = xdiv<STRICT>(torque_out, gr);
         if (vel_u < 1e-9 && torque_t < 0) torque_t = 0;
-        dcrank = u.inv_ci * (drive_torque(P, tqs, throttle, crank) + torque_in);
+        dcrank = u.inv_ci * (drive_torque<STRICT>(P, tqs, throttle, crank) + torque_in);
         shift_w = omega_out;
     } else {
-        crank = qdiv(0.25 * osum, gr);
-        torque_t = qdiv(drive_torque(P, tqs, throttle, crank), gr);
+        crank = xdiv<STRICT>(0.25 * osum, gr);
+        torque_t = xdiv<STRICT>(drive_torque<STRICT>(P, tqs, throttle, crank), gr);
         if (vel_u < 1e-9 && torque_t < 0) torque_t = 0;
         shift_w = crank;
     }
@@ -423,8 +426,8 @@ LFDM_HD void powertrain4(const PSet& P, const Derived& u, double tqs, double max
         tf = torque_t;
         trr = 0;
     }
-    diff_split(tf, omega[0], omega[1], eng[0], eng[1]);
-    diff_split(trr, omega[2], omega[3], eng[2], eng[3]);
+    diff_split<STRICT>(tf, omega[0], omega[1], eng[0], eng[1]);
+    diff_split<STRICT>(trr, omega[2], omega[3], eng[2], eng[3]);
     const double bt = maxBrake * braking;
 #pragma unroll
     for (int i = 0; i < 4; ++i)

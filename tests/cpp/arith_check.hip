@@ -1,14 +1,19 @@
 // GPU check of the step code's quotient and square root (qdiv, qsqrt in csrc/dev_common.h) against
-// the compiler's IEEE `/` and sqrt on the device and against the host's correctly rounded results.
+// the compiler's IEEE `/` and sqrt on the device and against the host's correctly rounded results, and
+// of the fast form's faithful quotient, reciprocal and square root (fdiv, frcp, fsqrt): each within one
+// ulp of the correctly rounded result.
 //
 // Operands: random signs, mantissas and exponents over the range the helpers are specified for
 // (quotients of operands with exponents in [-450, 450], square roots of x in [2^-700, 2^700]), plus
 // operands shaped like the step's (ratios of numbers in [1e-6, 1e6], zero numerators, squares of
 // zero).  Prints the mismatch counts; exits non-zero if qdiv or qsqrt differs from the device's `/` or
-// sqrt (the device-versus-host counts are information).
+// sqrt or if fdiv, frcp or fsqrt is more than one ulp from the correctly rounded result (the
+// device-versus-host counts, the number of faithful results that are not the correctly rounded one and
+// the largest relative error of the hardware estimates v_rcp_f64 and v_rsq_f64 are information).
 // Usage: arith_check [n]
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -18,6 +23,9 @@
 
 #include "../../low-fidelity-dynamic-models_amd/csrc/dev_common.h"
 
+using lfdm::dev::fdiv;
+using lfdm::dev::frcp;
+using lfdm::dev::fsqrt;
 using lfdm::dev::qdiv;
 using lfdm::dev::qsqrt;
 
@@ -29,6 +37,29 @@ __global__ void arith(const double* a, const double* b, const double* x, int n, 
     q_fast[i] = qdiv(a[i], b[i]);
     s_ieee[i] = sqrt(x[i]);
     s_fast[i] = qsqrt(x[i]);
+}
+
+// out[5 n]: fdiv(a, b), frcp(b), fsqrt(x), v_rcp_f64(b), v_rsq_f64(x)
+__global__ void faithful(const double* a, const double* b, const double* x, int n, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = fdiv(a[i], b[i]);
+    out[n + i] = frcp(b[i]);
+    out[2 * n + i] = fsqrt(x[i]);
+    out[3 * n + i] = __builtin_amdgcn_rcp(b[i]);
+    out[4 * n + i] = __builtin_amdgcn_rsq(x[i]);
+}
+
+// Distance in ulps between two finite doubles of the same sign (0 and -0 are 0 apart).
+static long long ulps(double p, double q) {
+    if (p == q) return 0;
+    if ((p < 0) != (q < 0)) return 1LL << 62;
+    long long ip, iq;
+    std::memcpy(&ip, &p, 8);
+    std::memcpy(&iq, &q, 8);
+    ip &= 0x7fffffffffffffffLL;
+    iq &= 0x7fffffffffffffffLL;
+    return ip > iq ? ip - iq : iq - ip;
 }
 
 static bool same(double p, double q) { return std::memcmp(&p, &q, sizeof p) == 0; }
@@ -80,6 +111,14 @@ int main(int argc, char** argv) {
     hipLaunchKernelGGL(arith, dim3(grid), dim3(block), 0, 0, da, db, dx, n, q1, q2, s1, s2);
     CHECK(hipGetLastError());
     CHECK(hipDeviceSynchronize());
+    double* fo;
+    CHECK(hipMalloc(&fo, 5 * bytes));
+    hipLaunchKernelGGL(faithful, dim3(grid), dim3(block), 0, 0, da, db, dx, n, fo);
+    CHECK(hipGetLastError());
+    CHECK(hipDeviceSynchronize());
+    std::vector<double> hf(5 * (size_t)n);
+    CHECK(hipMemcpy(hf.data(), fo, 5 * bytes, hipMemcpyDeviceToHost));
+    CHECK(hipFree(fo));
     std::vector<double> hq1(n), hq2(n), hs1(n), hs2(n);
     CHECK(hipMemcpy(hq1.data(), q1, bytes, hipMemcpyDeviceToHost));
     CHECK(hipMemcpy(hq2.data(), q2, bytes, hipMemcpyDeviceToHost));
@@ -104,7 +143,36 @@ int main(int argc, char** argv) {
     std::printf("device '/' vs host: %ld mismatches\n", qh);
     std::printf("qsqrt vs device sqrt: %ld mismatches\n", sd);
     std::printf("device sqrt vs host: %ld mismatches\n", sh);
-    const bool ok = qd == 0 && sd == 0;
+    // faithful forms: within one ulp of the correctly rounded host result
+    long long fmax_[3] = {0, 0, 0}, fnotcr[3] = {0, 0, 0};
+    double rcp_err = 0, rsq_err = 0;
+    for (int i = 0; i < n; ++i) {
+        const double ref[3] = {a[i] / b[i], 1.0 / b[i], std::sqrt(x[i])};
+        for (int k = 0; k < 3; ++k) {
+            const long long u = ulps(hf[(size_t)k * n + i], ref[k]);
+            if (u > fmax_[k]) {
+                if (u > 1 && fmax_[k] <= 1)
+                    std::printf("%s(%a, %a) = %a, correctly rounded %a\n", k == 0 ? "fdiv" : k == 1 ? "frcp" : "fsqrt",
+                                k == 2 ? x[i] : a[i], b[i], hf[(size_t)k * n + i], ref[k]);
+                fmax_[k] = u;
+            }
+            if (u) ++fnotcr[k];
+        }
+        rcp_err = std::max(rcp_err, std::fabs(hf[3 * (size_t)n + i] * b[i] - 1.0));
+        if (x[i] > 0) {
+            const double y = hf[4 * (size_t)n + i];
+            rsq_err = std::max(rsq_err, std::fabs(y * std::sqrt(x[i]) - 1.0));
+        }
+    }
+    std::printf("fdiv: at most %lld ulp from the correctly rounded quotient; %lld not correctly rounded\n", fmax_[0],
+                fnotcr[0]);
+    std::printf("frcp: at most %lld ulp from the correctly rounded reciprocal; %lld not correctly rounded\n", fmax_[1],
+                fnotcr[1]);
+    std::printf("fsqrt: at most %lld ulp from the correctly rounded square root; %lld not correctly rounded\n",
+                fmax_[2], fnotcr[2]);
+    std::printf("hardware estimates: v_rcp_f64 relative error <= %.3g (2^%.1f), v_rsq_f64 <= %.3g (2^%.1f)\n", rcp_err,
+                std::log2(rcp_err), rsq_err, std::log2(rsq_err));
+    const bool ok = qd == 0 && sd == 0 && fmax_[0] <= 1 && fmax_[1] <= 1 && fmax_[2] <= 1;
     std::printf("%s\n", ok ? "ALL OK" : "MISMATCH");
     return ok ? 0 : 1;
 }
