@@ -62,7 +62,8 @@ LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, doubl
     // vehToTireTransform (dof11.cpp:45-71)
     const double delta = P.nonLinearSteer ? map_y<STRICT>(P.steer, steering) : steering * P.maxSteer;
     double sd, cd;
-    sincos(delta, &sd, &cd);
+    if (STRICT) sincos(delta, &sd, &cd);
+    else sincos_fast(delta, sd, cd);
     double vsx[2], vsy[2];
     vsy[0] = s.v + s.wz * P.a;
     vsx[0] = s.u * cd + vsy[0] * sd;
@@ -176,7 +177,8 @@ LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, doubl
     s.v = s.v + vdot * dt;
     s.wz = s.wz + wzdot * dt;
     double sp, cp;
-    sincos(s.psi, &sp, &cp);
+    if (STRICT) sincos(s.psi, &sp, &cp);
+    else sincos_fast(s.psi, sp, cp);
     s.x = s.x + (s.u * cp - s.v * sp) * dt;
     s.y = s.y + (s.u * sp + s.v * cp) * dt;
     s.psi = s.psi + s.wz * dt;

@@ -540,7 +540,7 @@ LFDM_HD void step18_adj(const PSet& P, const Derived& u, const K18& k, const S18
     for (int i = 0; i < 4; ++i) fz[i] = fzr[i] > 0. ? fzr[i] : 0.;
     const double delta = steering * P.maxSteer;
     double sd, cd;
-    sincos(delta, &sd, &cd);
+    sincos_fast(delta, sd, cd);
     double vsx[4], vsy[4];
     vsy[0] = s0.v + s0.wz * P.a;
     vsx[0] = (s0.u - (s0.wz * P.cf) / 2.) * cd + vsy[0] * sd;
@@ -607,7 +607,7 @@ LFDM_HD void step18_adj(const PSet& P, const Derived& u, const K18& k, const S18
     const double udot = s0.wz * s0.v + k.inv_mt * inner;
     const double un = s0.u + udot * dt, vn = s0.v + vdot * dt, wzn = s0.wz + wzdot * dt;
     double sp, cp;
-    sincos(s0.psi, &sp, &cp);
+    sincos_fast(s0.psi, sp, cp);
 
     // ================= reverse =================
     S18b o = sb;  // bars of the state after the step

@@ -130,7 +130,8 @@ LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, doubl
     // vehToTireTransform
     const double delta = P.nonLinearSteer ? map_y<STRICT>(P.steer, steering) : steering * P.maxSteer;
     double sd, cd;
-    sincos(delta, &sd, &cd);
+    if (STRICT) sincos(delta, &sd, &cd);
+    else sincos_fast(delta, sd, cd);
     double vsx[4], vsy[4];
     vsy[0] = s.v + s.wz * P.a;
     vsx[0] = (s.u - (s.wz * P.cf) / 2.) * cd + vsy[0] * sd;
@@ -207,7 +208,8 @@ LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, doubl
     s.wx = s.wx + wxdot * dt;
     s.wz = s.wz + wzdot * dt;
     double sp, cp;
-    sincos(s.psi, &sp, &cp);
+    if (STRICT) sincos(s.psi, &sp, &cp);
+    else sincos_fast(s.psi, sp, cp);
     s.x = s.x + (s.u * cp - s.v * sp) * dt;
     s.y = s.y + (s.u * sp + s.v * cp) * dt;
     s.psi = s.psi + s.wz * dt;

@@ -71,8 +71,13 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     const TireP& t = P.tire;
     const double a = P.a, b = P.b, cf = P.cf, cr = P.cr;
     double sth, cth, sph, cph;
-    sincos(s.theta, &sth, &cth);
-    sincos(s.phi, &sph, &cph);
+    if (STRICT) {
+        sincos(s.theta, &sth, &cth);
+        sincos(s.phi, &sph, &cph);
+    } else {
+        sincos_fast(s.theta, sth, cth);
+        sincos_fast(s.phi, sph, cph);
+    }
     // vehToSusTransform (dof24.cpp:148-230)
     double us[4], vs[4], ws[4], ls[4], uu[4], vu[4], dus[4], dvs[4], duu[4], dvu[4];
     us[0] = (-cf * s.wz / 2.) + s.u;
@@ -109,7 +114,8 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     // vehToTireTransform (dof24.cpp:313-385) with last step's static radius
     const double delta = P.nonLinearSteer ? map_y<STRICT>(P.steer, steering) : steering * P.maxSteer;
     double sd, cd;
-    sincos(delta, &sd, &cd);
+    if (STRICT) sincos(delta, &sd, &cd);
+    else sincos_fast(delta, sd, cd);
     double vgx[4], vgy[4], vsx[4], vsy[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -251,7 +257,8 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     s.psi = s.psi + dpsi * dt;
     s.phi = s.phi + dphi * dt;
     double sp, cp;
-    sincos(s.psi, &sp, &cp);
+    if (STRICT) sincos(s.psi, &sp, &cp);
+    else sincos_fast(s.psi, sp, cp);
     s.x = s.x + (s.u * cp - s.v * sp) * dt;
     s.y = s.y + (s.u * sp + s.v * cp) * dt;
     s.udot = udot;

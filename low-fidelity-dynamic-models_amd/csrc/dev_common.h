@@ -128,18 +128,100 @@ LFDM_HD double xdiv(double n, double d) {
     return STRICT ? qdiv(n, d) : fdiv(n, d);
 }
 
+// Sine and cosine of the fast arithmetic form, from the Taylor polynomials of sin and cos on
+// [-pi/4, pi/4] (to the r^17 and r^18 terms; truncation below 1e-19 there), evaluated by Horner's rule
+// in r^2.  Coefficients: (-1)^k / (2k+1)! and (-1)^k / (2k)!, rounded to double.
+LFDM_HD double sin_poly(double r, double z) {
+    double p = 0x1.952c77030ad4ap-49;
+    p = __builtin_fma(p, z, -0x1.ae7f3e733b81fp-41);
+    p = __builtin_fma(p, z, 0x1.6124613a86d09p-33);
+    p = __builtin_fma(p, z, -0x1.ae64567f544e4p-26);
+    p = __builtin_fma(p, z, 0x1.71de3a556c734p-19);
+    p = __builtin_fma(p, z, -0x1.a01a01a01a01ap-13);
+    p = __builtin_fma(p, z, 0x1.1111111111111p-7);
+    p = __builtin_fma(p, z, -0x1.5555555555555p-3);
+    return __builtin_fma(r * z, p, r);
+}
+
+LFDM_HD double cos_poly(double z) {
+    double p = -0x1.6827863b97d97p-53;
+    p = __builtin_fma(p, z, 0x1.ae7f3e733b81fp-45);
+    p = __builtin_fma(p, z, -0x1.93974a8c07c9dp-37);
+    p = __builtin_fma(p, z, 0x1.1eed8eff8d898p-29);
+    p = __builtin_fma(p, z, -0x1.27e4fb7789f5cp-22);
+    p = __builtin_fma(p, z, 0x1.a01a01a01a01ap-16);
+    p = __builtin_fma(p, z, -0x1.6c16c16c16c17p-10);
+    p = __builtin_fma(p, z, 0x1.5555555555555p-5);
+    p = __builtin_fma(p, z, -0.5);
+    return __builtin_fma(z, p, 1.0);
+}
+
+// sin(a) and cos(a) of the fast form: Cody-Waite reduction a = k pi/2 + r with pi/2 in two parts
+// (hi, the double nearest pi/2, and mid, the double nearest pi/2 - hi; the first fused step is exact
+// and the neglected third part contributes below |k| 1e-32), the polynomials above, and
+// the quadrant's swap and signs.  Within about one ulp of the correctly rounded results for |a| below
+// 2^20 pi/2 (~1.6e6); larger arguments go to the device libm.  About half the instructions of the
+// libm's sincos, which reduces in double-double arithmetic.
+LFDM_HD void sincos_fast(double a, double& s, double& c) {
+    if (!(fabs(a) < 0x1.921fb54442d18p+20)) {
+        sincos(a, &s, &c);
+        return;
+    }
+    const double k = __builtin_rint(a * 0x1.45f306dc9c883p-1);  // a * 2/pi
+    double r = __builtin_fma(-k, 0x1.921fb54442d18p+0, a);
+    r = __builtin_fma(-k, 0x1.1a62633145c07p-54, r);
+    const double z = r * r;
+    const double sr = sin_poly(r, z), cr = cos_poly(z);
+    const int q = (int)k;
+    const bool swap = (q & 1) != 0;
+    const double s0 = swap ? cr : sr, c0 = swap ? sr : cr;
+    s = (q & 2) ? -s0 : s0;
+    c = ((q + 1) & 2) ? -c0 : c0;
+}
+
+// sin(2 pi v) of the fast form, for the sine-step weights, whose angle is 2 pi times a number v in
+// [0, 1]: v - rint(v), then +-1/2 - r where |r| > 1/4 (periodicity and sin(pi - a) = sin(a); both
+// subtractions are exact), leave t in [-1/4, 1/4], and the Taylor polynomial of sin(2 pi t) to its
+// t^21 term (coefficients (-1)^k (2 pi)^(2k+1) / (2k+1)!, rounded to double; truncation below 2e-18
+// for |2 pi t| <= pi/2) is evaluated by Horner's rule in t^2, its leading coefficient 2 pi in two parts
+// (hi, the double nearest 2 pi, enters the last fused step exactly; lo = 2 pi - hi joins the rest).
+// The reference's angle is the rounded product k2Pi * v, so the two differ by that rounding (below
+// 1e-15); about a third of the libm's instructions, with no reduction branch.
+LFDM_HD double sin_2pi_unit(double v) {
+    const double r = v - __builtin_rint(v);
+    const double t = fabs(r) > 0.25 ? (r < 0 ? -0.5 : 0.5) - r : r;
+    const double z = t * t;
+    double p = 0x1.2877020d52cf0p-10;
+    p = __builtin_fma(p, z, -0x1.8a404211f9547p-7);
+    p = __builtin_fma(p, z, 0x1.aaec32af93359p-4);
+    p = __builtin_fma(p, z, -0x1.6fadb9f155744p-1);
+    p = __builtin_fma(p, z, 0x1.e8f434d018d63p+1);
+    p = __builtin_fma(p, z, -0x1.e3074fde8871fp+3);
+    p = __builtin_fma(p, z, 0x1.50783487ee782p+5);
+    p = __builtin_fma(p, z, -0x1.32d2cce62bd86p+6);
+    p = __builtin_fma(p, z, 0x1.466bc6775aae2p+6);
+    p = __builtin_fma(p, z, -0x1.4abbce625be53p+5);
+    p = __builtin_fma(p, z, 0x1.1a62633145c07p-52);
+    return __builtin_fma(t, 0x1.921fb54442d18p+2, t * p);
+}
+
 LFDM_HD double sgn(double v) { return (double)((0.0 < v) - (v < 0.0)); }
 
 LFDM_HD double clampd(double v, double lo, double hi) {
     return v < lo ? lo : (v > hi ? hi : v);
 }
 
-// sineStep (utils.cpp:128-138) with x1, y1, x2, y2 = 1, 1, 1.5, 0: the tire-force weighting.
+// sineStep (utils.cpp:128-138) with x1, y1, x2, y2 = 1, 1, 1.5, 0: the tire-force weighting.  The
+// fast form takes the sine of k2Pi v, v = (x - 1) / dx (exact: x - 1 by Sterbenz's lemma, then a
+// doubling), as sin_2pi_unit(v).
+template <bool STRICT>
 LFDM_HD double sine_step_weight(double x) {
     if (x <= 1.0) return 1.0;
     if (x >= 1.5) return 0.0;
     const double dx = 0.5, dy = -1.0;
-    return 1.0 + dy * (x - 1.0) / dx - (dy / k2Pi) * sin(k2Pi * (x - 1.0) / dx);
+    if (STRICT) return 1.0 + dy * (x - 1.0) / dx - (dy / k2Pi) * sin(k2Pi * (x - 1.0) / dx);
+    const double v = (x - 1.0) / dx;
+    return 1.0 + dy * v - (dy / k2Pi) * sin_2pi_unit(v);
 }
 
 // General sineStep (utils.cpp:128-138).
@@ -148,7 +230,9 @@ LFDM_HD double sine_step(double x, double x1, double y1, double x2, double y2) {
     if (x <= x1) return y1;
     if (x >= x2) return y2;
     const double dx = x2 - x1, dy = y2 - y1;
-    return y1 + xdiv<STRICT>(dy * (x - x1), dx) - xdiv<STRICT>(dy, k2Pi) * sin(xdiv<STRICT>(k2Pi * (x - x1), dx));
+    if (STRICT) return y1 + qdiv(dy * (x - x1), dx) - qdiv(dy, k2Pi) * sin(qdiv(k2Pi * (x - x1), dx));
+    // the fast form's sine of 2 pi (x - x1) / dx, with (x - x1) / dx in [0, 1]
+    return y1 + fdiv(dy * (x - x1), dx) - fdiv(dy, k2Pi) * sin_2pi_unit(fdiv(x - x1, dx));
 }
 
 // Clamped piecewise-linear lookup (utils.cpp:109-125). `m` is wave-uniform, so the knot scan

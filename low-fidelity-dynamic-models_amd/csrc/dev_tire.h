@@ -241,8 +241,8 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
     w.fystr = clampd(w.fyr, -t.fymP2n, t.fymP2n);
     w.avx = fabs(w.vsx2);
     w.avy = fabs(w.vy);
-    w.wxw = sine_step_weight(w.avx);
-    w.wyw = sine_step_weight(w.avy);
+    w.wxw = sine_step_weight<STRICT>(w.avx);
+    w.wyw = sine_step_weight<STRICT>(w.avy);
     w.fx = w.wxw * w.fxstr + (1. - w.wxw) * w.fxdyn;
     w.fy = w.wyw * w.fystr + (1. - w.wyw) * w.fydyn;
 }

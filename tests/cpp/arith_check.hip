@@ -1,7 +1,9 @@
 // GPU check of the step code's quotient and square root (qdiv, qsqrt in csrc/dev_common.h) against
 // the compiler's IEEE `/` and sqrt on the device and against the host's correctly rounded results, and
 // of the fast form's faithful quotient, reciprocal and square root (fdiv, frcp, fsqrt): each within one
-// ulp of the correctly rounded result.
+// ulp of the correctly rounded result; and of the fast form's sine and cosine (sincos_fast over angles
+// up to 1e5 rad, sin_2pi_unit over [0, 1]): within two ulps of the correctly rounded results (host long
+// double sinl / cosl, rounded to double).
 //
 // Operands: random signs, mantissas and exponents over the range the helpers are specified for
 // (quotients of operands with exponents in [-450, 450], square roots of x in [2^-700, 2^700]), plus
@@ -48,6 +50,17 @@ __global__ void faithful(const double* a, const double* b, const double* x, int 
     out[2 * n + i] = fsqrt(x[i]);
     out[3 * n + i] = __builtin_amdgcn_rcp(b[i]);
     out[4 * n + i] = __builtin_amdgcn_rsq(x[i]);
+}
+
+// out[3 n]: sincos_fast(a) (sine, cosine), sin_2pi_unit(v)
+__global__ void trig(const double* a, const double* v, int n, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s, c;
+    lfdm::dev::sincos_fast(a[i], s, c);
+    out[i] = s;
+    out[n + i] = c;
+    out[2 * n + i] = lfdm::dev::sin_2pi_unit(v[i]);
 }
 
 // Distance in ulps between two finite doubles of the same sign (0 and -0 are 0 apart).
@@ -119,6 +132,26 @@ int main(int argc, char** argv) {
     std::vector<double> hf(5 * (size_t)n);
     CHECK(hipMemcpy(hf.data(), fo, 5 * bytes, hipMemcpyDeviceToHost));
     CHECK(hipFree(fo));
+    // angles: steering-sized (|a| <= 0.6), a few turns (|a| <= 20), yaw after long runs (|a| <= 1e5);
+    // sine-step arguments v in [0, 1]
+    std::vector<double> ang(n), unit(n);
+    std::uniform_real_distribution<double> u06(-0.6, 0.6), u20(-20.0, 20.0), u1e5(-1e5, 1e5), u01(0.0, 1.0);
+    for (int i = 0; i < n; ++i) {
+        ang[i] = i % 3 == 0 ? u06(rng) : i % 3 == 1 ? u20(rng) : u1e5(rng);
+        unit[i] = i % 1024 == 0 ? (double)(i / 1024 % 5) * 0.25 : u01(rng);  // include 0, 1/4, 1/2, 3/4, 1
+    }
+    double *dang, *dunit, *to;
+    CHECK(hipMalloc(&dang, bytes));
+    CHECK(hipMalloc(&dunit, bytes));
+    CHECK(hipMalloc(&to, 3 * bytes));
+    CHECK(hipMemcpy(dang, ang.data(), bytes, hipMemcpyHostToDevice));
+    CHECK(hipMemcpy(dunit, unit.data(), bytes, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(trig, dim3(grid), dim3(block), 0, 0, dang, dunit, n, to);
+    CHECK(hipGetLastError());
+    CHECK(hipDeviceSynchronize());
+    std::vector<double> ht(3 * (size_t)n);
+    CHECK(hipMemcpy(ht.data(), to, 3 * bytes, hipMemcpyDeviceToHost));
+    for (double* p : {dang, dunit, to}) CHECK(hipFree(p));
     std::vector<double> hq1(n), hq2(n), hs1(n), hs2(n);
     CHECK(hipMemcpy(hq1.data(), q1, bytes, hipMemcpyDeviceToHost));
     CHECK(hipMemcpy(hq2.data(), q2, bytes, hipMemcpyDeviceToHost));
@@ -172,7 +205,31 @@ int main(int argc, char** argv) {
                 fmax_[2], fnotcr[2]);
     std::printf("hardware estimates: v_rcp_f64 relative error <= %.3g (2^%.1f), v_rsq_f64 <= %.3g (2^%.1f)\n", rcp_err,
                 std::log2(rcp_err), rsq_err, std::log2(rsq_err));
-    const bool ok = qd == 0 && sd == 0 && fmax_[0] <= 1 && fmax_[1] <= 1 && fmax_[2] <= 1;
+    // fast sine and cosine against the host's long double results rounded to double
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    long long tmax[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        // sin(2 pi v) = sin(2 pi t) for the exactly reduced t of sin_2pi_unit (v - rint(v), then +-1/2 - r),
+        // whose long double product with 2 pi has no cancellation near the zeros
+        const double r = unit[i] - std::rint(unit[i]);
+        const double t = std::fabs(r) > 0.25 ? (r < 0 ? -0.5 : 0.5) - r : r;
+        const double ref[3] = {(double)sinl((long double)ang[i]), (double)cosl((long double)ang[i]),
+                               (double)sinl(two_pi * (long double)t)};
+        for (int k = 0; k < 3; ++k) {
+            const long long u = ulps(ht[(size_t)k * n + i], ref[k]);
+            if (u > tmax[k]) {
+                if (u > 2 && tmax[k] <= 2)
+                    std::printf("%s(%a) = %a, correctly rounded %a\n", k == 0 ? "sin" : k == 1 ? "cos" : "sin_2pi_unit",
+                                k == 2 ? unit[i] : ang[i], ht[(size_t)k * n + i], ref[k]);
+                tmax[k] = u;
+            }
+        }
+    }
+    std::printf("sincos_fast: sine at most %lld ulp, cosine at most %lld ulp from the correctly rounded values\n",
+                tmax[0], tmax[1]);
+    std::printf("sin_2pi_unit: at most %lld ulp from the correctly rounded value\n", tmax[2]);
+    const bool ok = qd == 0 && sd == 0 && fmax_[0] <= 1 && fmax_[1] <= 1 && fmax_[2] <= 1 && tmax[0] <= 2 &&
+                    tmax[1] <= 2 && tmax[2] <= 2;
     std::printf("%s\n", ok ? "ALL OK" : "MISMATCH");
     return ok ? 0 : 1;
 }
