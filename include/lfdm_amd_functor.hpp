@@ -10,7 +10,9 @@
 // Needs a HIP compiler (hipcc): the library's integration kernel is instantiated here on the functor's type
 // and launched on the handle's stream with the launch record the library prepares
 // (lfdm_functor_chunk_begin / _done); nothing is allocated and nothing synchronizes.  Compile with
-// -ffp-contract=off, as the library is, so the step keeps the reference's operation order.
+// -ffp-contract=off, as the library is, so the step keeps the reference's operation order, and with
+// -mllvm -disable-machine-licm, as the library is, so the step's constants are not held in vector
+// registers across the step loop (without it the kernel is correct but spills more).
 #pragma once
 #include <hip/hip_runtime.h>
 

@@ -27,9 +27,10 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// Waves per SIMD of the segment recompute (two: the group's (segment, vehicle) lanes fill them).
+// Waves per SIMD of the segment recompute (three: its 164 VGPRs fit without spills, and the group's
+// (segment, vehicle) lanes fill them).
 #ifndef LFDM_ADJ_RECOMPUTE_WAVES
-#define LFDM_ADJ_RECOMPUTE_WAVES 2
+#define LFDM_ADJ_RECOMPUTE_WAVES 3
 #endif
 
 static_assert(sizeof(K18) % 16 == 8 && sizeof(K18b) % 16 == 8, "odd number of doubles per lane in LDS");

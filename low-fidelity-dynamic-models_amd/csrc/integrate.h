@@ -95,7 +95,7 @@ struct M18 {
     using X = X18;
     using K = K18;
     static constexpr int kCh = 13;
-    static constexpr int kWaves = 2;
+    static constexpr int kWaves = kMaxWaves18;
     template <bool NR>
     struct R {
         static constexpr int kTire = NR ? 11 : 15, kVeh = 4 * kTire, om = NR ? 0 : 4;
@@ -167,7 +167,7 @@ struct M11 {
     using X = X11;
     using K = K11;
     static constexpr int kCh = 9;
-    static constexpr int kWaves = 3;
+    static constexpr int kWaves = kMaxWaves11;
     template <bool NR>
     struct R {
         static constexpr int kTire = NR ? 11 : 15, kVeh = 2 * kTire, om = NR ? 0 : 4;
@@ -239,7 +239,7 @@ struct M24 {
     using X = X24;
     using K = K24;
     static constexpr int kCh = 15;
-    static constexpr int kWaves = 1;
+    static constexpr int kWaves = kMaxWaves24;
     template <bool NR>
     struct R {
         static constexpr int kTire = NR ? 19 : 23, kSus = 4 * kTire, kVeh = kSus + 4 * 24, om = NR ? 0 : 4;
@@ -338,11 +338,11 @@ struct M24 {
 
 // Occupancy of the integration kernels (waves per SIMD).  Each model has a default, M::kWaves, the
 // most waves per SIMD it is built for: on large fleets more resident waves hide more FP64 latency
-// than the register spills of a smaller budget cost, up to three for the 11-DOF step (228 B of
-// spills, +17 % over two) and two for the 18-DOF step (140 B; three would spill 520 B and run 4 %
-// slower); the 24-DOF step would spill over a kilobyte per lane at two waves and runs 2.6x faster
-// at one.  A fleet that fills at most one wavefront per SIMD gains nothing from the other waves'
-// register budget, so every kernel also exists at one wave per SIMD, and the 11-DOF one at two;
+// than the register spills of a smaller budget cost, up to four for the 11-DOF step (292 B of spills
+// per lane, +4.5 % over three) and three for the 18-DOF step (436 B, +10 % over two; four would spill
+// 596 B and run 17 % slower than three); the 24-DOF step would spill over a kilobyte per lane at two
+// waves and runs 1.2x faster at one.  A fleet that fills at most one wavefront per SIMD gains nothing
+// from the other waves' register budget, so every kernel also exists at every lower occupancy;
 // Launch::waves picks (0: the model's default).  STRICT instantiations evaluate the step with the
 // reference's own quotients and transcendentals (Launch::strict; built at the model's default
 // occupancy only).
@@ -438,6 +438,9 @@ template <class M>
 hipError_t launch_model(const Launch& L, int block, hipStream_t stream) {
     if (L.strict) return launch_tire<M, M::kWaves, true>(L, block, stream);
     const int w = L.waves == 0 ? M::kWaves : L.waves;
+    if constexpr (M::kWaves >= 4) {
+        if (w == 4) return launch_tire<M, 4>(L, block, stream);
+    }
     if constexpr (M::kWaves >= 3) {
         if (w == 3) return launch_tire<M, 3>(L, block, stream);
     }

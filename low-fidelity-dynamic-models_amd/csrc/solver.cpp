@@ -1490,7 +1490,7 @@ int fleet_check(lfdm_fleet* f) {
 }
 
 // Occupancies the model's integration kernels are built for (24-DOF: one wave per SIMD only).
-int max_waves(int dof) { return dof == 24 ? 1 : (dof == 18 ? 2 : 3); }
+int max_waves(int dof) { return dof == 24 ? kMaxWaves24 : (dof == 18 ? kMaxWaves18 : kMaxWaves11); }
 
 void fleet_drop_graph(lfdm_fleet* f) {
     if (f->exec) (void)hipGraphExecDestroy(f->exec);

@@ -48,7 +48,7 @@ def test_fleet_arguments():
     with pytest.raises(LfdmError) as e:  # a handle is one bucket
         f.add(s)
     assert e.value.name == "LFDM_E_ARG"
-    for bad in ((1, 256, 0), (0, 96, 0), (0, 512, 0), (0, 256, -1), (0, 256, 0, 3)):
+    for bad in ((1, 256, 0), (0, 96, 0), (0, 512, 0), (0, 256, -1), (0, 256, 0, 4)):
         with pytest.raises(LfdmError) as e:
             f.set_config(*bad)
         assert e.value.name == "LFDM_E_ARG"
