@@ -65,8 +65,8 @@ struct FunctorControls {
 
 template <class M, bool NR, int W, bool STRICT, class F>
 hipError_t launch_functor_at(const lfdm::Launch& L, int block, int grid, hipStream_t st, const F& f) {
-    hipLaunchKernelGGL((lfdm::kern::integrate<M, NR, lfdm::kCtrlFunctor, W, STRICT, F>), dim3(grid), dim3(block), 0,
-                       st, L, f);
+    hipLaunchKernelGGL((lfdm::kern::integrate<M, NR, lfdm::kCtrlFunctor, W, STRICT, F>), dim3(grid), dim3(block),
+                       lfdm::kern::k_lds_bytes<M>(block), st, L, f);
     return hipGetLastError();
 }
 

@@ -96,6 +96,7 @@ struct M18 {
     using K = K18;
     static constexpr int kCh = 13;
     static constexpr int kWaves = kMaxWaves18;
+    static constexpr bool kKInLds = true;
     template <bool NR>
     struct R {
         static constexpr int kTire = NR ? 11 : 15, kVeh = 4 * kTire, om = NR ? 0 : 4;
@@ -168,6 +169,7 @@ struct M11 {
     using K = K11;
     static constexpr int kCh = 9;
     static constexpr int kWaves = kMaxWaves11;
+    static constexpr bool kKInLds = true;
     template <bool NR>
     struct R {
         static constexpr int kTire = NR ? 11 : 15, kVeh = 2 * kTire, om = NR ? 0 : 4;
@@ -240,6 +242,7 @@ struct M24 {
     using K = K24;
     static constexpr int kCh = 15;
     static constexpr int kWaves = kMaxWaves24;
+    static constexpr bool kKInLds = true;
     template <bool NR>
     struct R {
         static constexpr int kTire = NR ? 19 : 23, kSus = 4 * kTire, kVeh = kSus + 4 * 24, om = NR ? 0 : 4;
@@ -368,7 +371,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
     typename M::S s;
     M::template load<NR>(L.state, S, slot, s);
     const Derived& u = P.d;
-    typename M::K k;
+    // the per-vehicle constants: in this lane's LDS slot (M::kKInLds; the launch sizes the dynamic LDS
+    // with k_lds_bytes), read where the step uses them, or in registers
+    extern __shared__ double lfdm_k_lds[];
+    typename M::K k_reg;
+    typename M::K& k = M::kKInLds ? reinterpret_cast<typename M::K*>(lfdm_k_lds)[threadIdx.x] : k_reg;
     M::kinit(P, L.ovr + slot, S, k);
     const int user = L.slot_user[slot];
     const int uv = user >= 0 ? user : 0;  // padding lanes read vehicle 0's controls
@@ -381,6 +388,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
     int next_out = L.n_out > 0 ? L.out_steps[0] : -1;
     // all steps but the lane's last keep only the carried state ...
     for (int kk = 0; kk < wsteps - 1; ++kk) {
+        if constexpr (M::kKInLds) LFDM_RELOAD_MEMORY();
         if (kk < my - 1) {
             const Ctl c = controls_at<MODE>(L, P, cs, uv, kk, t, f);
             M::template step<NR, false, STRICT>(P, u, k, s, c, dt, nullptr);
@@ -416,14 +424,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
     }
 }
 
+// Dynamic LDS of an integration launch of `block` threads: one K per lane where the model keeps its
+// per-vehicle constants in LDS.
+template <class M>
+constexpr size_t k_lds_bytes(int block) {
+    return M::kKInLds ? (size_t)block * sizeof(typename M::K) : 0;
+}
+
 template <class M, bool NR, int W, bool STRICT>
 hipError_t launch_mode(const Launch& L, int block, hipStream_t stream) {
     const int grid = (L.S + block - 1) / block;
+    const size_t lds = k_lds_bytes<M>(block);
     switch (L.mode) {
-        case kCtrlTable: hipLaunchKernelGGL((integrate<M, NR, kCtrlTable, W, STRICT>), grid, block, 0, stream, L, NoControls{}); break;
-        case kCtrlConst: hipLaunchKernelGGL((integrate<M, NR, kCtrlConst, W, STRICT>), grid, block, 0, stream, L, NoControls{}); break;
-        case kCtrlKnots: hipLaunchKernelGGL((integrate<M, NR, kCtrlKnots, W, STRICT>), grid, block, 0, stream, L, NoControls{}); break;
-        case kCtrlSteps: hipLaunchKernelGGL((integrate<M, NR, kCtrlSteps, W, STRICT>), grid, block, 0, stream, L, NoControls{}); break;
+        case kCtrlTable: hipLaunchKernelGGL((integrate<M, NR, kCtrlTable, W, STRICT>), grid, block, lds, stream, L, NoControls{}); break;
+        case kCtrlConst: hipLaunchKernelGGL((integrate<M, NR, kCtrlConst, W, STRICT>), grid, block, lds, stream, L, NoControls{}); break;
+        case kCtrlKnots: hipLaunchKernelGGL((integrate<M, NR, kCtrlKnots, W, STRICT>), grid, block, lds, stream, L, NoControls{}); break;
+        case kCtrlSteps: hipLaunchKernelGGL((integrate<M, NR, kCtrlSteps, W, STRICT>), grid, block, lds, stream, L, NoControls{}); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
