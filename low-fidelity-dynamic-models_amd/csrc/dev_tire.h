@@ -9,24 +9,27 @@
 namespace lfdm {
 namespace dev {
 
-// tmxy_combined (dof18.cpp:247-288); returns fos, and f through `f`.  STRICT divides by the
-// adhesion-branch denominator twice, as the reference writes it; otherwise once, as a reciprocal.
+// tmxy_combined (dof18.cpp:247-288); returns fos, and f through `f`.  STRICT divides as the
+// reference writes it.  The fast form takes rs = 1 / s from the caller (the tire core has it, as the
+// reciprocal of the combined slip) and forms 1 / sm once, so the quotients by s and by sm become
+// products, and divides by the adhesion branch's denominator once, as a reciprocal.
 template <bool STRICT = false>
-LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, double fs,
-                                           double& f) {
-    double df0loc = 0.0;
+LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, double fs, double rs,
+                        double& f) {
+    double df0loc = 0.0, rsm = 0.0;
     if (sm > 0.0) {
-        const double c = xdiv<STRICT>(2.0 * fm, sm);
+        if (!STRICT) rsm = frcp(sm);
+        const double c = STRICT ? qdiv(2.0 * fm, sm) : 2.0 * fm * rsm;
         df0loc = (c < df0) ? df0 : c;
     }
     double fos;
     if (s > 0.0 && df0loc > 0.0) {
         if (s > ss) {
             f = fs;
-            fos = xdiv<STRICT>(f, s);
+            fos = STRICT ? qdiv(f, s) : f * rs;
         } else if (s < sm) {
             const double p = xdiv<STRICT>(df0loc * sm, fm) - 2.0;
-            const double sn = xdiv<STRICT>(s, sm);
+            const double sn = STRICT ? qdiv(s, sm) : s * rsm;
             const double dn = 1.0 + (sn + p) * sn;
             if (STRICT) {
                 f = xdiv<STRICT>(df0loc * sm * sn, dn);
@@ -37,7 +40,7 @@ LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, d
                 fos = df0loc * rdn;
             }
         } else {
-            const double q = xdiv<STRICT>(fm, sm);
+            const double q = STRICT ? qdiv(fm, sm) : fm * rsm;
             const double a = xdiv<STRICT>(q * q, df0loc * sm);
             const double sstar = sm + xdiv<STRICT>(fm - fs, a * (ss - sm));
             if (sstar <= ss) {
@@ -51,7 +54,7 @@ LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, d
                 const double sn = xdiv<STRICT>(s - sm, ss - sm);
                 f = fm - (fm - fs) * sn * sn * (3.0 - 2.0 * sn);
             }
-            fos = xdiv<STRICT>(f, s);
+            fos = STRICT ? qdiv(f, s) : f * rs;
         }
     } else {
         f = 0.0;
@@ -118,7 +121,7 @@ struct TireEval {
     bool clip;
     double fzc;
     Curve c;
-    double fzn, rx, ry, ssum, rsum, rssum, rrsum, hsxn, hsyn, rhsxn, rhsyn, sxn, syn, sc, ca, sa;
+    double fzn, rx, ry, ssum, rsum, rssum, rrsum, hsxn, hsyn, rhsxn, rhsyn, sxn, syn, sc, rsc, ca, sa;
     double A, B, C, D, E, F, G, H, I, J, df0, fm, sm, fs, ss, f, fos;
     double ssv, vtxs, vtys, vy, numx, denx, numy, deny, xedot, yedot;
     double fxdyn, fydyn, fxr, fyr, fxstr, fystr, avx, avy, wxw, wyw, fx, fy, My;
@@ -157,10 +160,18 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
     const Curve& c = w.c;
     curve_at<STRICT>(t, u, qy1, qy2, w.fzc, w.c);
     w.fzn = w.fzc * u.inv_pn;
-    w.rx = xdiv<STRICT>(c.fxm, c.dfx0);
-    w.ry = xdiv<STRICT>(c.fym, c.dfy0);
+    if (STRICT) {
+        w.rx = qdiv(c.fxm, c.dfx0);
+        w.ry = qdiv(c.fym, c.dfy0);
+    } else {
+        // the fast form's pairs of quotients share one reciprocal: 1 / a = b / (a b), 1 / b = a / (a b)
+        const double rd = frcp(c.dfx0 * c.dfy0);
+        w.rx = c.fxm * c.dfy0 * rd;
+        w.ry = c.fym * c.dfx0 * rd;
+    }
     w.ssum = c.sxm + c.sym;
     w.rsum = w.rx + w.ry;
+    w.rsc = 0.;
     if (STRICT) {
         w.hsxn = qdiv(c.sxm, w.ssum) + qdiv(w.rx, w.rsum);
         w.hsyn = qdiv(c.sym, w.ssum) + qdiv(w.ry, w.rsum);
@@ -175,19 +186,21 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
             w.sa = kHalfSqrt2;
         }
     } else {
-        w.rssum = frcp(w.ssum);
-        w.rrsum = frcp(w.rsum);
+        const double rp = frcp(w.ssum * w.rsum);
+        w.rssum = w.rsum * rp;
+        w.rrsum = w.ssum * rp;
         w.hsxn = c.sxm * w.rssum + w.rx * w.rrsum;
         w.hsyn = c.sym * w.rssum + w.ry * w.rrsum;
-        w.rhsxn = frcp(w.hsxn);
-        w.rhsyn = frcp(w.hsyn);
+        const double rh = frcp(w.hsxn * w.hsyn);
+        w.rhsxn = w.hsyn * rh;
+        w.rhsyn = w.hsxn * rh;
         w.sxn = w.sx * w.rhsxn;
         w.syn = w.sy * w.rhsyn;
         w.sc = hyp(w.sxn, w.syn);
         if (w.sc > 0) {
-            const double rsc = frcp(w.sc);
-            w.ca = w.sxn * rsc;
-            w.sa = w.syn * rsc;
+            w.rsc = frcp(w.sc);
+            w.ca = w.sxn * w.rsc;
+            w.sa = w.syn * w.rsc;
         } else {
             w.ca = kHalfSqrt2;
             w.sa = kHalfSqrt2;
@@ -218,7 +231,7 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
     w.sm = hyp2<STRICT>(w.E, w.F);
     w.fs = hyp2<STRICT>(w.G, w.H);
     w.ss = hyp2<STRICT>(w.I, w.J);
-    w.fos = tmxy_fos<STRICT>(w.sc, w.df0, w.sm, w.fm, w.ss, w.fs, w.f);
+    w.fos = tmxy_fos<STRICT>(w.sc, w.df0, w.sm, w.fm, w.ss, w.fs, w.rsc, w.f);
     // -sineStep(vta, 0, 0, 0, 1) * rr * fz * rStat * sgn(omega) with the limited fz; the step is 0
     // for vta <= 0, else 1
     w.ssv = (w.vta <= 0.0) ? 0.0 : 1.0;
@@ -230,8 +243,14 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
     w.denx = w.vtxs * t.dx + w.fos;
     w.numy = -w.vtys * t.cy * ye - w.fos * w.vy;
     w.deny = w.vtys * t.dy + w.fos;
-    w.xedot = xdiv<STRICT>(w.numx, w.denx);
-    w.yedot = xdiv<STRICT>(w.numy, w.deny);
+    if (STRICT) {
+        w.xedot = qdiv(w.numx, w.denx);
+        w.yedot = qdiv(w.numy, w.deny);
+    } else {
+        const double rde = frcp(w.denx * w.deny);
+        w.xedot = w.numx * w.deny * rde;
+        w.yedot = w.numy * w.denx * rde;
+    }
     // the reference forms dx * num / den as (dx * num) / den here, but dy * (num / den) below
     w.fxdyn = (STRICT ? qdiv(t.dx * w.numx, w.denx) : t.dx * w.xedot) + t.cx * xe;
     w.fydyn = t.dy * w.yedot + (t.cy * ye);
@@ -314,7 +333,7 @@ LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double
     const double fs = t.mu * hyp2<STRICT>(c.fxs * ca, c.fys * sa);
     const double ss = t.mu * hyp2<STRICT>(c.sxs * ca, c.sys * sa);
     double f;
-    tmxy_fos<STRICT>(sc, df0, sm, fm, ss, fs, f);
+    tmxy_fos<STRICT>(sc, df0, sm, fm, ss, fs, rsc, f);
     double Fx, Fy;
     if (sc > 0. && STRICT) {
         Fx = qdiv(f * sx, sc);
