@@ -10,17 +10,18 @@ namespace lfdm {
 namespace dev {
 
 // tmxy_combined (dof18.cpp:247-288); returns fos, and f through `f`.  STRICT divides as the
-// reference writes it.  The fast form takes rs = 1 / s from the caller (the tire core has it, as the
-// reciprocal of the combined slip) and forms 1 / sm once, so the quotients by s and by sm become
-// products, and divides by the adhesion branch's denominator once, as a reciprocal.
+// reference writes it.  The fast form takes the reciprocals rs = 1 / s, rdf0 = 1 / df0, rfm = 1 / fm
+// and rsm = 1 / sm from the caller (the tire core forms each with its square root), so that the
+// quotients by s, sm, fm and df0loc become products, and divides by the adhesion branch's denominator
+// once, as a reciprocal.  The strict form ignores them.
 template <bool STRICT = false>
-LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, double fs, double rs,
-                        double& f) {
-    double df0loc = 0.0, rsm = 0.0;
+LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, double fs, double rs, double rdf0,
+                        double rfm, double rsm, double& f) {
+    double df0loc = 0.0, rdf0loc = 0.0;
     if (sm > 0.0) {
-        if (!STRICT) rsm = frcp(sm);
         const double c = STRICT ? qdiv(2.0 * fm, sm) : 2.0 * fm * rsm;
         df0loc = (c < df0) ? df0 : c;
+        rdf0loc = (c < df0) ? rdf0 : 0.5 * sm * rfm;  // 1 / df0loc
     }
     double fos;
     if (s > 0.0 && df0loc > 0.0) {
@@ -28,7 +29,7 @@ LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, d
             f = fs;
             fos = STRICT ? qdiv(f, s) : f * rs;
         } else if (s < sm) {
-            const double p = xdiv<STRICT>(df0loc * sm, fm) - 2.0;
+            const double p = (STRICT ? qdiv(df0loc * sm, fm) : df0loc * sm * rfm) - 2.0;
             const double sn = STRICT ? qdiv(s, sm) : s * rsm;
             const double dn = 1.0 + (sn + p) * sn;
             if (STRICT) {
@@ -41,7 +42,7 @@ LFDM_HD double tmxy_fos(double s, double df0, double sm, double fm, double ss, d
             }
         } else {
             const double q = STRICT ? qdiv(fm, sm) : fm * rsm;
-            const double a = xdiv<STRICT>(q * q, df0loc * sm);
+            const double a = STRICT ? qdiv(q * q, df0loc * sm) : q * q * rdf0loc * rsm;
             const double sstar = sm + xdiv<STRICT>(fm - fs, a * (ss - sm));
             if (sstar <= ss) {
                 if (s <= sstar) {
@@ -150,10 +151,12 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
         w.sx = qdiv(-w.vsx2, w.vta);
         w.sy = -tan(atan2(vsy, w.vta) - delta);
     } else {
-        const double rvta = frcp(w.vta);
-        w.sx = -w.vsx2 * rvta;
-        const double r = vsy * rvta;  // tan(atan2(vsy, vta))
-        w.sy = -fdiv(r - tan_delta, 1. + r * tan_delta);
+        // sy = -(r - tan d) / (1 + r tan d), r = vsy / vta = tan(atan2(vsy, vta)), is
+        // -(vsy - vta tan d) / (vta + vsy tan d); sx and sy share one reciprocal
+        const double den = w.vta + vsy * tan_delta;
+        const double rq = frcp(w.vta * den);
+        w.sx = -w.vsx2 * den * rq;
+        w.sy = -(vsy - w.vta * tan_delta) * w.vta * rq;
     }
     w.clip = fz > t.pnmax;
     w.fzc = w.clip ? t.pnmax : fz;
@@ -172,6 +175,7 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
     w.ssum = c.sxm + c.sym;
     w.rsum = w.rx + w.ry;
     w.rsc = 0.;
+    double rdf0 = 0., rfm = 0., rsm = 0.;
     if (STRICT) {
         w.hsxn = qdiv(c.sxm, w.ssum) + qdiv(w.rx, w.rsum);
         w.hsyn = qdiv(c.sym, w.ssum) + qdiv(w.ry, w.rsum);
@@ -196,9 +200,10 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
         w.rhsyn = w.hsxn * rh;
         w.sxn = w.sx * w.rhsxn;
         w.syn = w.sy * w.rhsyn;
-        w.sc = hyp(w.sxn, w.syn);
+        double rsc;
+        fsqrt_rcp(w.sxn * w.sxn + w.syn * w.syn, w.sc, rsc);
         if (w.sc > 0) {
-            w.rsc = frcp(w.sc);
+            w.rsc = rsc;
             w.ca = w.sxn * w.rsc;
             w.sa = w.syn * w.rsc;
         } else {
@@ -226,12 +231,18 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
         w.I = c.sxs * w.ca * w.rhsxn;
         w.J = c.sys * w.sa * w.rhsyn;
     }
-    w.df0 = hyp2<STRICT>(w.A, w.B);
-    w.fm = hyp2<STRICT>(w.C, w.D);
-    w.sm = hyp2<STRICT>(w.E, w.F);
+    if (STRICT) {
+        w.df0 = hypot(w.A, w.B);
+        w.fm = hypot(w.C, w.D);
+        w.sm = hypot(w.E, w.F);
+    } else {
+        fsqrt_rcp(w.A * w.A + w.B * w.B, w.df0, rdf0);
+        fsqrt_rcp(w.C * w.C + w.D * w.D, w.fm, rfm);
+        fsqrt_rcp(w.E * w.E + w.F * w.F, w.sm, rsm);
+    }
     w.fs = hyp2<STRICT>(w.G, w.H);
     w.ss = hyp2<STRICT>(w.I, w.J);
-    w.fos = tmxy_fos<STRICT>(w.sc, w.df0, w.sm, w.fm, w.ss, w.fs, w.rsc, w.f);
+    w.fos = tmxy_fos<STRICT>(w.sc, w.df0, w.sm, w.fm, w.ss, w.fs, w.rsc, rdf0, rfm, rsm, w.f);
     // -sineStep(vta, 0, 0, 0, 1) * rr * fz * rStat * sgn(omega) with the limited fz; the step is 0
     // for vta <= 0, else 1
     w.ssv = (w.vta <= 0.0) ? 0.0 : 1.0;
@@ -247,9 +258,8 @@ LFDM_HD void tmeasy_eval(const TireP& t, const Derived& u, double qy1, double qy
         w.xedot = qdiv(w.numx, w.denx);
         w.yedot = qdiv(w.numy, w.deny);
     } else {
-        const double rde = frcp(w.denx * w.deny);
-        w.xedot = w.numx * w.deny * rde;
-        w.yedot = w.numy * w.denx * rde;
+        w.xedot = fdiv(w.numx, w.denx);
+        w.yedot = fdiv(w.numy, w.deny);
     }
     // the reference forms dx * num / den as (dx * num) / den here, but dy * (num / den) below
     w.fxdyn = (STRICT ? qdiv(t.dx * w.numx, w.denx) : t.dx * w.xedot) + t.cx * xe;
@@ -308,18 +318,20 @@ LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double
         sx = qdiv(-vsx, vta);
         sy = -tan(atan2(vsy, vta) - delta);
     } else {
-        const double rvta = frcp(vta);
-        sx = -vsx * rvta;
-        const double r = vsy * rvta;  // tan(atan2(vsy, vta))
-        sy = -fdiv(r - tan_delta, 1. + r * tan_delta);
+        // as in tmeasy_eval: -(vsy - vta tan d) / (vta + vsy tan d), one reciprocal with sx
+        const double den = vta + vsy * tan_delta;
+        const double rq = frcp(vta * den);
+        sx = -vsx * den * rq;
+        sy = -(vsy - vta * tan_delta) * vta * rq;
     }
     if (fz > t.pnmax) fz = t.pnmax;
     Curve c;
     curve_at<STRICT>(t, u, qy1, qy2, fz, c);
     const double frblend = sine_step<STRICT>(fabs(vsx_in), t.frblend_begin, 0., t.frblend_end, 1.);
-    const double sc = hyp2<STRICT>(sx, sy);
+    double sc, rsc = 0.;
+    if (STRICT) sc = hypot(sx, sy);
+    else fsqrt_rcp(sx * sx + sy * sy, sc, rsc);
     double ca, sa;
-    const double rsc = (!STRICT && sc > 0) ? frcp(sc) : 0.;
     if (sc > 0) {
         ca = STRICT ? qdiv(sx, sc) : sx * rsc;
         sa = STRICT ? qdiv(sy, sc) : sy * rsc;
@@ -327,13 +339,27 @@ LFDM_HD void tmeasy_nr_core(const TireP& t, const Derived& u, double qy1, double
         ca = kHalfSqrt2;
         sa = kHalfSqrt2;
     }
-    const double df0 = hyp2<STRICT>(c.dfx0 * ca, c.dfy0 * sa);
-    const double fm = t.mu * hyp2<STRICT>(c.fxm * ca, c.fym * sa);
-    const double sm = t.mu * hyp2<STRICT>(c.sxm * ca, c.sym * sa);
+    double df0, fm, sm, rdf0 = 0., rfm = 0., rsm = 0.;
+    if (STRICT) {
+        df0 = hypot(c.dfx0 * ca, c.dfy0 * sa);
+        fm = t.mu * hypot(c.fxm * ca, c.fym * sa);
+        sm = t.mu * hypot(c.sxm * ca, c.sym * sa);
+    } else {
+        const double ax = c.dfx0 * ca, ay = c.dfy0 * sa, bx = c.fxm * ca, by = c.fym * sa, ex = c.sxm * ca,
+                     ey = c.sym * sa;
+        double hf, hs;
+        fsqrt_rcp(ax * ax + ay * ay, df0, rdf0);
+        fsqrt_rcp(bx * bx + by * by, hf, rfm);
+        fsqrt_rcp(ex * ex + ey * ey, hs, rsm);
+        fm = t.mu * hf;
+        sm = t.mu * hs;
+        rfm = rfm * u.inv_mu;
+        rsm = rsm * u.inv_mu;
+    }
     const double fs = t.mu * hyp2<STRICT>(c.fxs * ca, c.fys * sa);
     const double ss = t.mu * hyp2<STRICT>(c.sxs * ca, c.sys * sa);
     double f;
-    tmxy_fos<STRICT>(sc, df0, sm, fm, ss, fs, rsc, f);
+    tmxy_fos<STRICT>(sc, df0, sm, fm, ss, fs, rsc, rdf0, rfm, rsm, f);
     double Fx, Fy;
     if (sc > 0. && STRICT) {
         Fx = qdiv(f * sx, sc);
@@ -393,6 +419,9 @@ LFDM_HD void powertrain4(const PSet& P, const Derived& u, double tqs, double max
     const double osum = omega[0] + omega[1] + omega[2] + omega[3];
     const double gr = pick(P.gears, P.ngears, gear);
     double shift_w;
+    // the quotients by the gear ratio stay divisions in the fast form too: the converter's speed ratio
+    // omega_out / crank is exactly 1 at a crank speed set from the same quotient, and the model's
+    // reverse-flow branch (sr > 1) flips with one ulp there
     if (P.tcbool) {
         const double omega_out = xdiv<STRICT>(0.25 * osum, gr);
         const double omega_in = crank;

@@ -44,7 +44,7 @@ struct Derived {
     double delta_rear;      // steering angle of the rear tires (steering input 0)
     double tan_delta_rear;  // its tangent
     // reciprocals and quotients of parameters
-    double inv_pn, inv_kt, inv_two_ab, krof_cf, kror_cr, brof_cf, bror_cr, inv_cf, inv_cr;
+    double inv_pn, inv_kt, inv_two_ab, krof_cf, kror_cr, brof_cf, bror_cr, inv_cf, inv_cr, inv_mu;
 };
 
 // Vehicle parameters of one set (11-DOF dof11.h:496-620, 18-DOF dof18.h:489-799, 24-DOF

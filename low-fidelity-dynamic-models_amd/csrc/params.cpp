@@ -401,6 +401,7 @@ void derive18(PSet& p) {
     u.tan_delta_rear = std::tan(u.delta_rear);
     u.inv_pn = 1. / t.pn;
     u.inv_kt = 1. / t.kt;
+    u.inv_mu = 1. / t.mu;
     u.inv_two_ab = 1. / u.two_ab;
     u.krof_cf = p.krof / p.cf;
     u.kror_cr = p.kror / p.cr;

@@ -3,7 +3,8 @@
 // of the fast form's faithful quotient, reciprocal and square root (fdiv, frcp, fsqrt): each within one
 // ulp of the correctly rounded result; and of the fast form's sine and cosine (sincos_fast over angles
 // up to 1e5 rad, sin_2pi_unit over [0, 1]): within two ulps of the correctly rounded results (host long
-// double sinl / cosl, rounded to double).
+// double sinl / cosl, rounded to double); and of fsqrt_rcp (square root and reciprocal square root from
+// one v_rsq_f64): each within one ulp.
 //
 // Operands: random signs, mantissas and exponents over the range the helpers are specified for
 // (quotients of operands with exponents in [-450, 450], square roots of x in [2^-700, 2^700]), plus
@@ -52,8 +53,8 @@ __global__ void faithful(const double* a, const double* b, const double* x, int 
     out[4 * n + i] = __builtin_amdgcn_rsq(x[i]);
 }
 
-// out[3 n]: sincos_fast(a) (sine, cosine), sin_2pi_unit(v)
-__global__ void trig(const double* a, const double* v, int n, double* out) {
+// out[5 n]: sincos_fast(a) (sine, cosine), sin_2pi_unit(v), fsqrt_rcp(x) (square root, its reciprocal)
+__global__ void trig(const double* a, const double* v, const double* x, int n, double* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s, c;
@@ -61,6 +62,10 @@ __global__ void trig(const double* a, const double* v, int n, double* out) {
     out[i] = s;
     out[n + i] = c;
     out[2 * n + i] = lfdm::dev::sin_2pi_unit(v[i]);
+    double g, rg;
+    lfdm::dev::fsqrt_rcp(x[i], g, rg);
+    out[3 * n + i] = g;
+    out[4 * n + i] = rg;
 }
 
 // Distance in ulps between two finite doubles of the same sign (0 and -0 are 0 apart).
@@ -143,15 +148,18 @@ int main(int argc, char** argv) {
     double *dang, *dunit, *to;
     CHECK(hipMalloc(&dang, bytes));
     CHECK(hipMalloc(&dunit, bytes));
-    CHECK(hipMalloc(&to, 3 * bytes));
+    CHECK(hipMalloc(&to, 5 * bytes));
     CHECK(hipMemcpy(dang, ang.data(), bytes, hipMemcpyHostToDevice));
     CHECK(hipMemcpy(dunit, unit.data(), bytes, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(trig, dim3(grid), dim3(block), 0, 0, dang, dunit, n, to);
+    double* dxs;
+    CHECK(hipMalloc(&dxs, bytes));
+    CHECK(hipMemcpy(dxs, x.data(), bytes, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(trig, dim3(grid), dim3(block), 0, 0, dang, dunit, dxs, n, to);
     CHECK(hipGetLastError());
     CHECK(hipDeviceSynchronize());
-    std::vector<double> ht(3 * (size_t)n);
-    CHECK(hipMemcpy(ht.data(), to, 3 * bytes, hipMemcpyDeviceToHost));
-    for (double* p : {dang, dunit, to}) CHECK(hipFree(p));
+    std::vector<double> ht(5 * (size_t)n);
+    CHECK(hipMemcpy(ht.data(), to, 5 * bytes, hipMemcpyDeviceToHost));
+    for (double* p : {dang, dunit, to, dxs}) CHECK(hipFree(p));
     std::vector<double> hq1(n), hq2(n), hs1(n), hs2(n);
     CHECK(hipMemcpy(hq1.data(), q1, bytes, hipMemcpyDeviceToHost));
     CHECK(hipMemcpy(hq2.data(), q2, bytes, hipMemcpyDeviceToHost));
@@ -228,8 +236,22 @@ int main(int argc, char** argv) {
     std::printf("sincos_fast: sine at most %lld ulp, cosine at most %lld ulp from the correctly rounded values\n",
                 tmax[0], tmax[1]);
     std::printf("sin_2pi_unit: at most %lld ulp from the correctly rounded value\n", tmax[2]);
+    // sqrt and reciprocal square root of fsqrt_rcp (the reciprocal for x > 0; long double reference)
+    long long smax[2] = {0, 0};
+    for (int i = 0; i < n; ++i) {
+        const double g = ht[3 * (size_t)n + i], rg = ht[4 * (size_t)n + i];
+        const long long us = ulps(g, std::sqrt(x[i]));
+        const long long ur = x[i] > 0 ? ulps(rg, (double)(1.0L / sqrtl((long double)x[i]))) : 0;
+        if ((us > 1 && smax[0] <= 1) || (ur > 1 && smax[1] <= 1))
+            std::printf("fsqrt_rcp(%a) = %a, %a; correctly rounded %a, %a\n", x[i], g, rg, std::sqrt(x[i]),
+                        (double)(1.0L / sqrtl((long double)x[i])));
+        smax[0] = std::max(smax[0], us);
+        smax[1] = std::max(smax[1], ur);
+    }
+    std::printf("fsqrt_rcp: square root at most %lld ulp, reciprocal at most %lld ulp from the correctly rounded values\n",
+                smax[0], smax[1]);
     const bool ok = qd == 0 && sd == 0 && fmax_[0] <= 1 && fmax_[1] <= 1 && fmax_[2] <= 1 && tmax[0] <= 2 &&
-                    tmax[1] <= 2 && tmax[2] <= 2;
+                    tmax[1] <= 2 && tmax[2] <= 2 && smax[0] <= 1 && smax[1] <= 1;
     std::printf("%s\n", ok ? "ALL OK" : "MISMATCH");
     return ok ? 0 : 1;
 }
