@@ -86,3 +86,36 @@ def test_host_step_forms_vs_oracle(dof, veh, tire, nr, sus):
     print(F.summary_line(f"{dof}-DOF {veh} {tire}, fast form", v))
     assert v["ok"], v["failing"]
     assert not np.array_equal(fast, R.traj)
+
+
+@pytest.mark.parametrize("dof", [18, 11, 24])
+def test_host_step_forms_at_rest_golden(dof):
+    """The reference's step_api case (standing still for 0.5 s, then steering 0.1 and throttle 0.5) through the
+    host step code, against the reference's trajectory (golden fixture), both forms within 1e-9 and the strict
+    form bit for bit.  Standing still, every force of the reference model is exactly 0 (the powertrain's loss
+    map passes through the origin, the tire deflections stay 0), and the rolling-resistance moment flips with
+    the sign of the wheel speed: a form that leaves a rounding residue where the reference has an exact 0
+    (a reciprocal segment width in a map lookup does) starts that flip and departs at O(1)."""
+    import cases18
+    import cases1124
+
+    L = host_lib()
+    g = np.load(os.path.join(ROOT, "tests", "golden", f"d{dof}_golden.npz"), allow_pickle=False)
+    name = "hmmwv_step_api"
+    if dof == 18:
+        flat, ctrl, n, _ = cases18.inputs(name, g)
+        init = np.zeros((1, O.record_len(18, False)))
+    else:
+        flat, ctrl, n, _, init = cases1124.inputs(dof, name, g)
+        init = np.ascontiguousarray(init.reshape(1, -1))
+    nsteps, every = (int(v) for v in g[f"{name}/spec"][:2])
+    c = np.ascontiguousarray(ctrl[1])
+    ovr = W.default_overrides(flat, 1, dof=dof)
+    ref = g[f"{name}/traj"]
+    for strict in (True, False):
+        tr = host_sampled(L, dof, False, strict, flat, init, c, ovr, every)
+        err = cases18.normwise_rel(tr, ref.reshape(tr.shape))
+        print(f"{dof}-DOF {name}, {'strict' if strict else 'fast'} form: max normwise rel {err.max():.3e}")
+        if strict:
+            np.testing.assert_array_equal(tr, ref.reshape(tr.shape))
+        assert err.max() <= F.TOL
