@@ -245,8 +245,9 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     s.wy = s.wy + wydot * dt;
     s.wz = s.wz + wzdot * dt;
     // Euler-angle rates at the step's (old) theta and phi: tan(theta) and 1/cos(theta) from the
-    // step's sine and cosine (same values, other rounding than tan() and two divisions)
-    const double rcth = frcp(cth);
+    // step's sine and cosine (same values, other rounding than tan() and two divisions); the fast
+    // form's 1/cos(theta) is cos(phi) / (cos(theta) cos(phi)), from the tires' reciprocal
+    const double rcth = cph * rcc;
     const double tth = STRICT ? tan(s.theta) : sth * rcth;
     const double dtheta = s.wy * cph - s.wz * sph;
     const double cth_e = STRICT ? cos(s.theta) : cth;
