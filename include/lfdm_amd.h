@@ -206,13 +206,18 @@ int lfdm_adjoint_forward(lfdm_solver* h, int nsteps, int checkpoint_every, const
                          double* device_xy);
 int lfdm_adjoint_backward(lfdm_solver* h, const double* device_xy_bar, double* device_ctrl_bar,
                           double* device_state_bar, double* device_param_bar);
-/* Device memory the adjoint may use for its tape (default 16 GiB), from the next
- * lfdm_adjoint_forward on.  The backward walks the checkpoint segments in groups, recomputing a whole
+/* Device memory the adjoint may use for its tape, from the next lfdm_adjoint_forward on; 0 (the
+ * default) is automatic: half of the device memory free when the forward starts (counting the tape the
+ * handle already holds).  The backward walks the checkpoint segments in groups, recomputing a whole
  * group's steps at once (38 doubles per vehicle and step; the forward tapes the last group itself);
- * a group holds as many segments as this budget does, at least one.  A budget that holds the whole
+ * a group holds as many segments as the budget does, at least one.  A budget that holds the whole
  * horizon (steps x 38 x total x 8 bytes) makes the forward tape every step and the backward recompute
  * nothing.  Results do not depend on it. */
 int lfdm_adjoint_set_tape_budget(lfdm_solver* h, unsigned long long bytes);
+/* Bytes of tape the last lfdm_adjoint_forward set up and its segments per backward group (the whole
+ * horizon's segments when the backward recomputes nothing); LFDM_E_STATE before any forward.  No
+ * counterpart in the reference (which has no adjoint). */
+int lfdm_adjoint_tape_bytes(lfdm_solver* h, unsigned long long* bytes, int* segments_per_group);
 /* Counter of the handle's adjoint state: it changes with every lfdm_adjoint_forward and whenever a
  * pending adjoint is cancelled (new per-vehicle parameters, a new slot layout).  A caller holding a
  * forward's value can tell whether its backward is still the one that forward set up. */

@@ -162,7 +162,7 @@ struct lfdm_solver {
     unsigned long long adj_gen = 0;  // adjoint forwards run on this handle (and pending ones invalidated)
     int adj_G = 1;                 // segments per backward group, fixed by the forward
     bool tape_holds_last = false;  // the tape holds the last group, written by the forward
-    size_t adj_tape_budget = size_t(16) << 30;  // bytes the backward's tape may take
+    size_t adj_tape_budget = 0;  // bytes the backward's tape may take (0: automatic, adj_budget)
     int simds = 0;  // SIMDs of the device (4 per CU), queried once
     const double* adj_ctrl = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1350,12 +1350,22 @@ AdjLaunch adj_launch(lfdm_solver* h) {
 }
 
 
-// Segments per backward group: as many as the handle's tape budget holds, at least one and at most
+// The tape budget of a forward: the handle's, or, left automatic, half of the device memory free at
+// the forward (counting the tape the handle already holds), so that on a 288 GB device the tape of
+// tens of thousands of vehicles over thousands of steps fits whole.
+size_t adj_budget(lfdm_solver* h) {
+    if (h->adj_tape_budget) return h->adj_tape_budget;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return size_t(16) << 30;
+    return (free_b + h->tape_cap * sizeof(double)) / 2;
+}
+
+// Segments per backward group: as many as the forward's tape budget holds, at least one and at most
 // all.  The forward tapes the group the backward walks first, so every segment beyond the first group
 // costs one recompute; with the whole horizon in the budget the backward recomputes nothing.
 int adj_group(lfdm_solver* h, int nseg) {
     const size_t per_seg = (size_t)h->adj_K * kAdjTapeRow * (size_t)h->S * sizeof(double);
-    const size_t G = h->adj_tape_budget / std::max<size_t>(per_seg, 1);
+    const size_t G = adj_budget(h) / std::max<size_t>(per_seg, 1);
     return (int)std::max<size_t>(1, std::min<size_t>(G, (size_t)nseg));
 }
 
@@ -1398,6 +1408,16 @@ int lfdm_adjoint_forward(lfdm_solver* h, int nsteps, int checkpoint_every, const
 int lfdm_adjoint_set_tape_budget(lfdm_solver* h, unsigned long long bytes) {
     if (int rc = check(h)) return rc;
     h->adj_tape_budget = (size_t)bytes;
+    return LFDM_OK;
+}
+
+int lfdm_adjoint_tape_bytes(lfdm_solver* h, unsigned long long* bytes, int* segments_per_group) {
+    if (int rc = check(h)) return rc;
+    if (!bytes) return fail(LFDM_E_ARG, "null output");
+    if (h->adj_nsteps <= 0) return fail(LFDM_E_STATE, "no adjoint forward has run on this handle");
+    *bytes = (unsigned long long)h->adj_G * (unsigned long long)h->adj_K * kAdjTapeRow * (unsigned long long)h->S *
+             sizeof(double);
+    if (segments_per_group) *segments_per_group = h->adj_G;
     return LFDM_OK;
 }
 

@@ -104,6 +104,7 @@ SIGNATURES = {
     "lfdm_adjoint_forward": [_p, _i, _i, _p, _p],
     "lfdm_adjoint_backward": [_p, _p, _p, _p, _p],
     "lfdm_adjoint_set_tape_budget": [_p, ctypes.c_ulonglong],
+    "lfdm_adjoint_tape_bytes": [_p, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_int)],
     "lfdm_step_jacobian": [_p, _p, _p, _p, _p],
     "lfdm_set_stream": [_p, _p],
     "lfdm_synchronize": [_p],

@@ -80,4 +80,5 @@ def run_mpc(solver, flat, init, params, ref, ctrl0, iterations=20, lr=0.05, chec
             ctrl[:, 0].clamp_(-1.0, 1.0)
             ctrl[:, 1:].clamp_(0.0, 1.0)
         history.append(loss.detach())
-    return ctrl.detach(), (torch.stack(history) / ctrl.shape[2]).tolist()
+    losses = (torch.stack(history) / ctrl.shape[2]).tolist() if history else []
+    return ctrl.detach(), losses

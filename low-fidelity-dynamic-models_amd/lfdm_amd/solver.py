@@ -300,7 +300,15 @@ class SolverHalfImplicitGPU:
                                         ctypes.c_void_p(state_bar_ptr), ctypes.c_void_p(param_bar_ptr)))
 
     def adjoint_set_tape_budget(self, nbytes):
+        """Device bytes the adjoint's tape may take from the next forward on; 0 (the default): half of
+        the device memory free at the forward."""
         check(lib.lfdm_adjoint_set_tape_budget(self._h, int(nbytes)))
+
+    def adjoint_tape_bytes(self):
+        """(bytes of tape, segments per backward group) of the last adjoint forward."""
+        b, g = ctypes.c_ulonglong(), ctypes.c_int()
+        check(lib.lfdm_adjoint_tape_bytes(self._h, ctypes.byref(b), ctypes.byref(g)))
+        return b.value, g.value
 
     def adjoint_generation(self):
         v = ctypes.c_ulonglong()

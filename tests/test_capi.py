@@ -130,6 +130,10 @@ def test_handle_lifecycle_errors(tmp_path):
     for bad_call in (lambda: s.SetTimeStep(0.0), lambda: s.SetKernelSimTime(-1), lambda: s.SetThreadsPerBlock(0)):
         with pytest.raises(LfdmError):
             bad_call()
+    s.adjoint_set_tape_budget(0)  # automatic (the default)
+    with pytest.raises(LfdmError) as e:  # no adjoint forward yet
+        s.adjoint_tape_bytes()
+    assert e.value.name == "LFDM_E_STATE"
     s.close()
     s2.close()
 

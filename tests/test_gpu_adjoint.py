@@ -266,15 +266,18 @@ def test_torch_rollout_backward_and_mpc_step():
     ctrl.grad = p.grad = i0.grad = None
     s.adjoint_set_tape_budget(1)
     xy = rollout(s, ctrl, p, i0, 100)
+    assert s.adjoint_tape_bytes() == (100 * 38 * n * 8, 1)
     tracking_loss(xy, ref).sum().backward()
     assert torch.equal(ctrl.grad, g1)
     gp1, gi1 = p.grad.clone(), i0.grad.clone()
     # groups of 3 of the 15 segments (the forward tapes the last group, the backward recomputes the
-    # other four), then the default budget, which holds all 15: the forward tapes every step
-    for budget in (3 * 100 * 38 * n * 8, 16 << 30):
+    # other four), then the automatic default (half the free device memory), which holds all 15: the
+    # forward tapes every step
+    for budget, groups in ((3 * 100 * 38 * n * 8, 3), (0, 15)):
         ctrl.grad = p.grad = i0.grad = None
         s.adjoint_set_tape_budget(budget)
         xy = rollout(s, ctrl, p, i0, 100)
+        assert s.adjoint_tape_bytes() == (groups * 100 * 38 * n * 8, groups)
         tracking_loss(xy, ref).sum().backward()
         assert torch.equal(ctrl.grad, g1) and torch.equal(p.grad, gp1) and torch.equal(i0.grad, gi1)
     say("[adjoint] MPC, 6 iterations")
