@@ -261,12 +261,14 @@ LFDM_HD double sine_step(double x, double x1, double y1, double x2, double y2) {
 
 // Clamped piecewise-linear lookup (utils.cpp:109-125). `m` is wave-uniform, so the knot scan
 // runs on scalar loads; the interval index counts knots strictly below x, which is exactly
-// std::lower_bound's answer on a sorted map.
+// std::lower_bound's answer on a sorted map.  Branch-free: the segment the scan picks is
+// interpolated for every x (the first segment at or below the first knot, the last at or above the
+// last knot; a one-knot map reads the zero entry after it, whose result is never selected) and the
+// end values are selected outside the knots, so a wavefront takes no divergent branch and its
+// scalar loads are not held behind one.
 template <bool STRICT>
 LFDM_HD double map_y(const Map& m, double x) {
     const int n = m.n;
-    if (x <= m.e[0].x) return m.e[0].y;
-    if (x >= m.e[n - 1].x) return m.e[n - 1].y;
     int r = 0;
     for (int k = 1; k < n - 1; ++k) r += (m.e[k].x < x) ? 1 : 0;
     r += 1;
@@ -275,7 +277,9 @@ LFDM_HD double map_y(const Map& m, double x) {
     // a division in both forms: a map through the origin gives exactly 0 at 0 (the powertrain's losses at
     // rest), which a reciprocal width would not
     const double mbar = xdiv<STRICT>(x - L.x, R.x - L.x);
-    return L.y + mbar * (R.y - L.y);
+    const double y = L.y + mbar * (R.y - L.y);
+    const MapXY F = m.e[0], B = m.e[n - 1];
+    return x <= F.x ? F.y : (x >= B.x ? B.y : y);
 }
 
 // Same lookup on a map whose abscissae are scaled by sx and ordinates by sy1 then sy2
@@ -283,8 +287,6 @@ LFDM_HD double map_y(const Map& m, double x) {
 template <bool STRICT>
 LFDM_HD double map_y_scaled(const Map& m, double x, double sx, double sy1, double sy2) {
     const int n = m.n;
-    if (x <= m.e[0].x * sx) return m.e[0].y * sy1 * sy2;
-    if (x >= m.e[n - 1].x * sx) return m.e[n - 1].y * sy1 * sy2;
     int r = 0;
     for (int k = 1; k < n - 1; ++k) r += (m.e[k].x * sx < x) ? 1 : 0;
     r += 1;
@@ -292,7 +294,9 @@ LFDM_HD double map_y_scaled(const Map& m, double x, double sx, double sy1, doubl
     const MapXY R = m.e[r];
     const double lx = L.x * sx, rx = R.x * sx, ly = L.y * sy1 * sy2, ry = R.y * sy1 * sy2;
     const double mbar = xdiv<STRICT>(x - lx, rx - lx);
-    return ly + mbar * (ry - ly);
+    const double y = ly + mbar * (ry - ly);
+    const MapXY F = m.e[0], B = m.e[n - 1];
+    return x <= F.x * sx ? F.y * sy1 * sy2 : (x >= B.x * sx ? B.y * sy1 * sy2 : y);
 }
 
 // Driver-table lookup at a wave-uniform time (utils.cpp:34-56).
