@@ -66,11 +66,12 @@ LFDM_HD double map_y_slope(const Map& m, double x, double& slope) {
     return L.y + mbar * (R.y - L.y);
 }
 
-// d sineStep(x, 1, 1, 1.5, 0) / dx
+// d sineStep(x, 1, 1, 1.5, 0) / dx; the cosine of the turn fraction v = (x - 1) / dx as the sine a
+// quarter turn on (sin_2pi_unit of v + 1/4, whose rounding moves the value by ~1e-16)
 LFDM_HD double sine_step_weight_d(double x) {
     if (x <= 1.0 || x >= 1.5) return 0.0;
     const double dx = 0.5, dy = -1.0;
-    return dy / dx - (dy / dx) * cos(k2Pi * (x - 1.0) / dx);
+    return dy / dx - (dy / dx) * sin_2pi_unit((x - 1.0) / dx + 0.25);
 }
 
 LFDM_HD void hypot_adj(double a, double b, double h, double hb, double& ab, double& bb) {
