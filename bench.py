@@ -218,6 +218,9 @@ def adjoint_cost(flat, n=65536, T=5000, K=100, mpc_iterations=20):
     torch.cuda.synchronize()
     p_ms = float(np.min(plain))
     n_inputs = 3 * T + 6 + 23  # controls, parameters (m, Izz, Ixx, cornering stiffness, torque scale, brake), state
+    if mpc_iterations:  # one untimed iteration: first-use allocations (gradients, optimizer state)
+        run_mpc(s, flat, init, params, ref, ctrl, iterations=1)
+        torch.cuda.synchronize()
     t0 = time.perf_counter()
     _, hist = run_mpc(s, flat, init, params, ref, ctrl, iterations=mpc_iterations)
     torch.cuda.synchronize()

@@ -68,7 +68,13 @@ def run_mpc(solver, flat, init, params, ref, ctrl0, iterations=20, lr=0.05, chec
     ctrl = ctrl0.clone().requires_grad_(True)
     params = params.to(ctrl.device, torch.float64)
     init = init.to(ctrl.device, torch.float64)
-    opt = torch.optim.Adam([ctrl], lr=lr)
+    try:  # one fused kernel per step over the (steps, 3, vehicles) controls where torch has it
+        opt = torch.optim.Adam([ctrl], lr=lr, fused=ctrl.is_cuda)
+    except (RuntimeError, TypeError):
+        opt = torch.optim.Adam([ctrl], lr=lr)
+    # steering within [-1, 1], throttle and braking within [0, 1], in one pass
+    lo = torch.tensor([-1.0, 0.0, 0.0], dtype=ctrl.dtype, device=ctrl.device).view(1, 3, 1)
+    hi = torch.ones((1, 3, 1), dtype=ctrl.dtype, device=ctrl.device)
     history = []
     for _ in range(iterations):
         opt.zero_grad()
@@ -77,8 +83,7 @@ def run_mpc(solver, flat, init, params, ref, ctrl0, iterations=20, lr=0.05, chec
         loss.backward()
         opt.step()
         with torch.no_grad():
-            ctrl[:, 0].clamp_(-1.0, 1.0)
-            ctrl[:, 1:].clamp_(0.0, 1.0)
+            torch.clamp(ctrl, lo, hi, out=ctrl)
         history.append(loss.detach())
     losses = (torch.stack(history) / ctrl.shape[2]).tolist() if history else []
     return ctrl.detach(), losses
