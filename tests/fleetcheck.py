@@ -12,7 +12,7 @@ normwise changes.  The bar, per vehicle:
   - spread <= STABLE (1e-11): the GPU trajectory is within TOL = 1e-9 of the oracle;
   - otherwise within max(TOL, SPREAD_MULT x spread): the GPU's difference is of the size of the
     model's own response to one ulp.  SPREAD_MULT = 25: on MI355X, over 10 s, the largest ratio of GPU
-    error to spread was 11 (over the 18-, 11- and 24-DOF TMeasy and TMeasyNr fleets, the mixed fleet and
+    error to spread was 12 (over the 18-, 11- and 24-DOF TMeasy and TMeasyNr fleets, the mixed fleet and
     both arithmetic forms; 99 % of the ill-conditioned vehicles stay below 7);
   - non-finite values (the reference model blows up for a few Sedan 24-DOF TMeasyNr vehicles) occur in
     the GPU trajectory exactly for the vehicles whose oracle trajectory has them, wherever that is
