@@ -6,6 +6,7 @@
 // must equal the oracle's (and so the reference CPU solver's) bit for bit, which pins the strict
 // form's formulas without a GPU.  The fast instantiation shows what the reformulations alone change.
 // Built as tests/cpp -> lfdm_amd/libhost_step.so (make hoststep); loaded by tests/test_strict_host.py.
+// Also exports the fast form's polynomial sine and cosine for CPU checks of their formulas.
 #include <cstring>
 #include <string>
 #include <vector>
@@ -83,6 +84,18 @@ int host_step_run(int dof, int nr, int strict, const double* flat, int len, cons
         return -1;
     }
     return 0;
+}
+
+// The fast form's sine and cosine as the host build of the step evaluates them (the polynomial code the
+// device runs, with the C library's fma): out[3 n] = sincos_fast(a) sine, cosine, then sin_2pi_unit(v).
+void host_fast_trig(const double* a, const double* v, int n, double* out) {
+    for (int i = 0; i < n; ++i) {
+        double s, c;
+        dev::sincos_fast(a[i], s, c);
+        out[i] = s;
+        out[n + i] = c;
+        out[2 * n + i] = dev::sin_2pi_unit(v[i]);
+    }
 }
 
 }  // extern "C"

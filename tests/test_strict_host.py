@@ -119,3 +119,39 @@ def test_host_step_forms_at_rest_golden(dof):
         if strict:
             np.testing.assert_array_equal(tr, ref.reshape(tr.shape))
         assert err.max() <= F.TOL
+
+
+def test_fast_sines_and_cosines_on_the_host():
+    """sincos_fast (Cody-Waite reduction, Taylor polynomials) and sin_2pi_unit (exact reduction of a turn
+    fraction) as the host build evaluates them, against x86 long double sinl / cosl rounded to double:
+    within two ulps over steering-sized, multi-turn and long-run (1e5 rad) angles and over [0, 1] turns,
+    exact at the quarter turns; NaN stays NaN.  (tests/test_gpu_arith.py checks the device build.)"""
+    L = host_lib()
+    L.host_fast_trig.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    rng = np.random.default_rng(9)
+    n = 30000
+    a = np.concatenate([rng.uniform(-0.6, 0.6, n // 3), rng.uniform(-20, 20, n // 3), rng.uniform(-1e5, 1e5, n // 3)])
+    v = np.concatenate([rng.uniform(0, 1, len(a) - 5), [0.0, 0.25, 0.5, 0.75, 1.0]])
+    out = np.empty(3 * len(a))
+    L.host_fast_trig(a.ctypes.data, v.ctypes.data, len(a), out.ctypes.data)
+    s, c, s2 = out[:len(a)], out[len(a):2 * len(a)], out[2 * len(a):]
+    ld = np.longdouble
+    ref_s = np.sin(a.astype(ld)).astype(np.float64)
+    ref_c = np.cos(a.astype(ld)).astype(np.float64)
+    # sin(2 pi v) = sin(2 pi t) for the exactly reduced t, whose long double product with 2 pi does not cancel
+    r = v - np.rint(v)
+    t = np.where(np.abs(r) > 0.25, np.where(r < 0, -0.5, 0.5) - r, r)
+    two_pi = ld(2) * ld(np.pi) + ld(2) * (ld(3.141592653589793238462643383279502884) - ld(np.pi))
+    ref_s2 = np.sin(two_pi * t.astype(ld)).astype(np.float64)
+
+    def ulps(x, ref):
+        return np.abs(x - ref) / np.spacing(np.abs(ref))
+
+    assert ulps(s, ref_s).max() <= 2 and ulps(c, ref_c).max() <= 2, (ulps(s, ref_s).max(), ulps(c, ref_c).max())
+    nz = ref_s2 != 0
+    assert ulps(s2[nz], ref_s2[nz]).max() <= 2
+    np.testing.assert_array_equal(s2[-5:], [0.0, 1.0, 0.0, -1.0, 0.0])
+    nan = np.array([np.nan])
+    o = np.empty(3)
+    L.host_fast_trig(nan.ctypes.data, nan.ctypes.data, 1, o.ctypes.data)
+    assert np.isnan(o).all()
