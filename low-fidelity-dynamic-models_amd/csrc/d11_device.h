@@ -48,7 +48,21 @@ LFDM_HD void k11_init(const PSet& P, double m, double jz, double dfy0Pn, double 
 
 // One half-implicit step (rhsFun order, dof11_halfImplicit.cpp:560-605; update :172-235).
 // STRICT: the reference's own quotients and transcendentals throughout (see dev_tire.h).
-template <bool NR, bool FULL, bool STRICT = false>
+// The step's last part, the position update from the new velocities at the step's yaw angle, then
+// the yaw angle (dof11_halfImplicit.cpp:214-219).  Nothing else in a step reads x, y or psi: a step loop may
+// leave it out (POS = false) and apply it at the start of the next step or before the state is read,
+// the same operations on the same values.
+template <bool STRICT = false>
+LFDM_HD void position11(S11& s, double dt) {
+    double sp, cp;
+    if (STRICT) sincos(s.psi, &sp, &cp);
+    else sincos_fast(s.psi, sp, cp);
+    s.x = s.x + (s.u * cp - s.v * sp) * dt;
+    s.y = s.y + (s.u * sp + s.v * cp) * dt;
+    s.psi = s.psi + s.wz * dt;
+}
+
+template <bool NR, bool FULL, bool STRICT = false, bool POS = true>
 LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, double steering,
                                        double throttle, double braking, double dt, X11* x) {
     const TireP& t = P.tire;
@@ -176,13 +190,8 @@ LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, doubl
     s.u = s.u + udot * dt;
     s.v = s.v + vdot * dt;
     s.wz = s.wz + wzdot * dt;
-    double sp, cp;
-    if (STRICT) sincos(s.psi, &sp, &cp);
-    else sincos_fast(s.psi, sp, cp);
-    s.x = s.x + (s.u * cp - s.v * sp) * dt;
-    s.y = s.y + (s.u * sp + s.v * cp) * dt;
-    s.psi = s.psi + s.wz * dt;
     s.udot = udot;
+    if (POS) position11<STRICT>(s, dt);
     if (FULL) {
 #pragma unroll
         for (int i = 0; i < 2; ++i) {

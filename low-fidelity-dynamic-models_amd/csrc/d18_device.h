@@ -107,7 +107,21 @@ struct TireOut18 {
 };
 
 // STRICT: the reference's own quotients and transcendentals throughout (see dev_tire.h).
-template <bool NR, bool FULL, bool CAPTURE = false, bool STRICT = false>
+// The step's last part, the position update from the new velocities at the step's yaw angle, then
+// the yaw angle (dof18_halfImplicit.cpp:301-305).  Nothing else in a step reads x, y or psi: a step
+// loop may leave it out (POS = false) and apply it at the start of the next step or before the state
+// is read, the same operations on the same values.
+template <bool STRICT = false>
+LFDM_HD void position18(S18& s, double dt) {
+    double sp, cp;
+    if (STRICT) sincos(s.psi, &sp, &cp);
+    else sincos_fast(s.psi, sp, cp);
+    s.x = s.x + (s.u * cp - s.v * sp) * dt;
+    s.y = s.y + (s.u * sp + s.v * cp) * dt;
+    s.psi = s.psi + s.wz * dt;
+}
+
+template <bool NR, bool FULL, bool CAPTURE = false, bool STRICT = false, bool POS = true>
 LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, double steering,
                                        double throttle, double braking, double dt, X18* x,
                                        TireOut18* to = nullptr) {
@@ -207,15 +221,10 @@ LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, doubl
     s.v = s.v + vdot * dt;
     s.wx = s.wx + wxdot * dt;
     s.wz = s.wz + wzdot * dt;
-    double sp, cp;
-    if (STRICT) sincos(s.psi, &sp, &cp);
-    else sincos_fast(s.psi, sp, cp);
-    s.x = s.x + (s.u * cp - s.v * sp) * dt;
-    s.y = s.y + (s.u * sp + s.v * cp) * dt;
-    s.psi = s.psi + s.wz * dt;
     s.phi = s.phi + s.wx * dt;
     s.udot = udot;
     s.vdot = vdot;
+    if (POS) position18<STRICT>(s, dt);
     if (FULL) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {

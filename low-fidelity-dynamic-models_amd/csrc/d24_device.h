@@ -58,6 +58,34 @@ LFDM_HD void k24_init(const PSet& P, double m, double jz, double jx, double dfy0
     k.half_cr = P.cr / 2.;
 }
 
+// vehToSusTransform (dof24.cpp:148-230) of strut i: strut-top velocities, strut length and
+// unsprung-mass velocities from the chassis velocities and the strut deflection.
+struct Kin24 {
+    double us, vs, ls, uu, vu;
+};
+
+LFDM_HD Kin24 kin24(const PSet& P, const S24& s, int i) {
+    const double a = P.a, b = P.b, cf = P.cf, cr = P.cr;
+    Kin24 q;
+    if (i == 0) {
+        q.us = (-cf * s.wz / 2.) + s.u;
+        q.vs = a * s.wz + s.v;
+    } else if (i == 1) {
+        q.us = (cf * s.wz / 2.) + s.u;
+        q.vs = a * s.wz + s.v;
+    } else if (i == 2) {
+        q.us = (-cr * s.wz / 2.) + s.u;
+        q.vs = -b * s.wz + s.v;
+    } else {
+        q.us = (cr * s.wz / 2.) + s.u;
+        q.vs = -b * s.wz + s.v;
+    }
+    q.ls = s.lsi[i] - (s.xs[i] - s.xsi[i]);
+    q.uu = q.us - q.ls * s.wy;
+    q.vu = q.vs + q.ls * s.wx;
+    return q;
+}
+
 // One half-implicit step (rhsFun order, dof24_halfImplicit.cpp:594-706; update :188-297).
 // STRICT: the reference's own quotients and transcendentals throughout (see dev_tire.h): the static
 // radius divided by cos(theta) cos(phi), the jacking forces by the track widths, the Euler-angle
@@ -65,7 +93,23 @@ LFDM_HD void k24_init(const PSet& P, double m, double jz, double jx, double dfy0
 // or only the cosine of an angle (computeTireRHS: cos theta and cos phi; computeForcesThroughSus:
 // sin phi; the Euler-angle rates of the step: cos theta), the strict step calls that function alone;
 // everywhere else the reference's sin and cos of one angle compile to one sincos, as here.
-template <bool NR, bool FULL, bool STRICT = false>
+// Each strut's kinematics (vehToSusTransform) are computed where a stage uses them: before its tire
+// and again after the tires, from the same state and with the same operations (the same values),
+// so they are not held in registers across the four tire evaluations.
+// The step's last part, the position update from the new velocities and yaw angle
+// (dof24_halfImplicit.cpp:275-280).  Nothing else in a step reads x and y, and psi only here: a step
+// loop may leave it out (POS = false) and apply it at the start of the next step or before the state
+// is read, the same operations on the same values.
+template <bool STRICT = false>
+LFDM_HD void position24(S24& s, double dt) {
+    double sp, cp;
+    if (STRICT) sincos(s.psi, &sp, &cp);
+    else sincos_fast(s.psi, sp, cp);
+    s.x = s.x + (s.u * cp - s.v * sp) * dt;
+    s.y = s.y + (s.u * sp + s.v * cp) * dt;
+}
+
+template <bool NR, bool FULL, bool STRICT = false, bool POS = true>
 LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, double steering,
                                        double throttle, double braking, double dt, X24* x) {
     const TireP& t = P.tire;
@@ -78,58 +122,25 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
         sincos_fast(s.theta, sth, cth);
         sincos_fast(s.phi, sph, cph);
     }
-    // vehToSusTransform (dof24.cpp:148-230)
-    double us[4], vs[4], ws[4], ls[4], uu[4], vu[4], dus[4], dvs[4], duu[4], dvu[4];
-    us[0] = (-cf * s.wz / 2.) + s.u;
-    vs[0] = a * s.wz + s.v;
-    ws[0] = (cf * s.wx / 2.) - (a * s.wy) + s.w;
-    us[1] = (cf * s.wz / 2.) + s.u;
-    vs[1] = a * s.wz + s.v;
-    ws[1] = (-cf * s.wx / 2.) - (a * s.wy) + s.w;
-    us[2] = (-cr * s.wz / 2.) + s.u;
-    vs[2] = -b * s.wz + s.v;
-    ws[2] = (cr * s.wx / 2.) + b * s.wy + s.w;
-    us[3] = (cr * s.wz / 2.) + s.u;
-    vs[3] = -b * s.wz + s.v;
-    ws[3] = (-cr * s.wx / 2.) + b * s.wy + s.w;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        ls[i] = s.lsi[i] - (s.xs[i] - s.xsi[i]);
-        uu[i] = us[i] - ls[i] * s.wy;
-        vu[i] = vs[i] + ls[i] * s.wx;
-    }
-    dus[0] = (-cf * s.wzdot) / 2 + s.udot;
-    dvs[0] = a * s.wzdot + s.vdot;
-    dus[1] = (cf * s.wzdot) / 2 + s.udot;
-    dvs[1] = a * s.wzdot + s.vdot;
-    dus[2] = (-cr * s.wzdot) / 2 + s.udot;
-    dvs[2] = -b * s.wzdot + s.vdot;
-    dus[3] = (cr * s.wzdot) / 2 + s.udot;
-    dvs[3] = (-b * s.wzdot) + s.vdot;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        duu[i] = dus[i] - (-s.dxs[i] * s.wy + ls[i] * s.wydot);
-        dvu[i] = dvs[i] + (-s.dxs[i] * s.wx + ls[i] * s.wxdot);
-    }
-    // vehToTireTransform (dof24.cpp:313-385) with last step's static radius
     const double delta = P.nonLinearSteer ? map_y<STRICT>(P.steer, steering) : steering * P.maxSteer;
     double sd, cd;
     if (STRICT) sincos(delta, &sd, &cd);
     else sincos_fast(delta, sd, cd);
-    double vgx[4], vgy[4], vsx[4], vsy[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        vgx[i] = cth * (uu[i] - s.wy * s.rStat[i]) + sth * (s.wu[i] * cph + sph * (s.wx * s.rStat[i] + vu[i]));
-        vgy[i] = cph * (vu[i] + s.wx * s.rStat[i]) - s.wu[i] * sph;
-        vsy[i] = vgy[i];
-        vsx[i] = i < 2 ? vgx[i] * cd + vgy[i] * sd : vgx[i];
-    }
+    // per corner: vehToTireTransform (dof24.cpp:313-385) with last step's static radius, then
     // computeTireRHS (dof24.cpp:827-1121): load from the compression, radius from the attitude
     double fx[4], fy[4], fz[4], My[4];
     const double cc = cth * cph;
     const double rcc = frcp(cc);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
+        const Kin24 q = kin24(P, s, i);
+        const double vgx = cth * (q.uu - s.wy * s.rStat[i]) + sth * (s.wu[i] * cph + sph * (s.wx * s.rStat[i] + q.vu));
+        const double vgy = cph * (q.vu + s.wx * s.rStat[i]) - s.wu[i] * sph;
+        const double vsy = vgy;
+        const double vsx = i < 2 ? vgx * cd + vgy * sd : vgx;
+        if (FULL) {
+            x->vgx[i] = vgx; x->vgy[i] = vgy; x->vsx[i] = vsx; x->vsy[i] = vsy;
+        }
         const double dl = i < 2 ? delta : u.delta_rear;
         const double d = i < 2 ? xdiv<STRICT>(sd, cd) : u.tan_delta_rear;
         fz[i] = s.xt[i] * t.kt;
@@ -140,104 +151,149 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
             continue;
         }
         if (!NR)
-            tmeasy_core<STRICT>(t, u, k.qy1, k.qy2, dl, d, fz[i], s.rStat[i], vsx[i], vsy[i], s.xe[i], s.ye[i],
+            tmeasy_core<STRICT>(t, u, k.qy1, k.qy2, dl, d, fz[i], s.rStat[i], vsx, vsy, s.xe[i], s.ye[i],
                                 s.omega[i], s.xedot[i], s.yedot[i], fx[i], fy[i], My[i]);
         else
-            tmeasy_nr_core<STRICT>(t, u, k.qy1, k.qy2, dl, d, fz[i], s.rStat[i], vsx[i], vsy[i], s.omega[i], fx[i],
+            tmeasy_nr_core<STRICT>(t, u, k.qy1, k.qy2, dl, d, fz[i], s.rStat[i], vsx, vsy, s.omega[i], fx[i],
                                    fy[i], My[i]);
     }
-    // computeTireCompressionVelocity (dof24.cpp:1122-1151)
-    double dxt[4];
+    // the strut kinematics again, recomputed rather than carried over the tires
+    opaque(s.wz);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) dxt[i] = s.vsz[i] - (cth * (s.wu[i] * cph + vu[i] * sph) - uu[i] * sth);
-    // computePowertrainRHS (dof24.cpp:1338-1509)
-    double dOm[4], eng[4], dcrank;
-    powertrain4<STRICT>(P, u, k.tqs, k.maxBrake, throttle, braking, s.u, s.crank, s.gear, s.omega, fx, s.rStat, My, dOm, eng,
-                dcrank);
-    // tireToVehTransform (dof24.cpp:459-528)
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const double gx = fx[i] * cd - fy[i] * sd;
-        const double gy = fx[i] * sd + fy[i] * cd;
-        fx[i] = gx;
-        fy[i] = gy;
-    }
-    double fxg[4], fyg[4], fzg[4];
+    for (int i = 0; i < 4; ++i) opaque(s.xs[i]);
+    double ls[4], uu[4], vu[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        fxg[i] = fx[i] * cth - fz[i] * sth;
-        fyg[i] = fx[i] * sph * sth + fy[i] * cph + fz[i] * sph * cth;
-        fzg[i] = fx[i] * sth * cph - fy[i] * sph + fz[i] * cph * cth;
+        const Kin24 q = kin24(P, s, i);
+        ls[i] = q.ls;
+        uu[i] = q.uu;
+        vu[i] = q.vu;
+        if (FULL) {
+            x->us[i] = q.us; x->vs[i] = q.vs; x->ls[i] = q.ls; x->uu[i] = q.uu; x->vu[i] = q.vu;
+        }
     }
-    // computeSusRHS (dof24.cpp:1182-1229): last step's fzd and dxs, then the new dxs
-    double dwu[4];
+    // computeTireCompressionVelocity (dof24.cpp:1122-1151).  Each state is advanced (the
+    // half-implicit update, dof24_halfImplicit.cpp:214-281) as soon as nothing later in the step reads
+    // its old value, with the same operations as an update at the end of the step.
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const int ax = i < 2 ? 0 : 1;
-        const double mu = ax == 0 ? P.muf : P.mur;
-        dwu[i] = k.inv_mu[ax] * (cph * (cth * (fz[i] - k.mu_g[ax]) + sth * fx[i]) - sph * fy[i] - s.fzd[i] -
-                                 s.xs[i] * P.ks - s.dxs[i] * P.bs - mu * (vu[i] * s.wx - uu[i] * s.wy));
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) s.dxs[i] = -ws[i] + s.wu[i];
-    // computeForcesThroughSus (dof24.cpp:599-680); the rear-right lateral force has no cos(theta)
-    // factor in the reference (:639) and keeps that form here
-    double fxs[4], fys[4], fzs[4];
-    const double sph_f = STRICT ? sin(s.phi) : sph;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int ax = i < 2 ? 0 : 1;
-        const double mu = ax == 0 ? P.muf : P.mur;
-        fxs[i] = fxg[i] + k.mu_g[ax] * sth - mu * duu[i] + mu * s.wz * vu[i] - mu * s.wy * s.wu[i];
-        if (i < 3)
-            fys[i] = fyg[i] - k.mu_g[ax] * sph_f * cth - mu * dvu[i] + mu * s.wx * s.wu[i] - mu * s.wz * uu[i];
-        else
-            fys[i] = fyg[i] - k.mu_g[ax] * sph_f - mu * dvu[i] + mu * s.wx * s.wu[i] - mu * s.wz * uu[i];
-        fzs[i] = s.xs[i] * P.ks + s.dxs[i] * P.bs;
-    }
-    const double jf = fyg[1] * s.rStat[1] + fys[1] * ls[1] + fyg[0] * s.rStat[0] + fys[0] * ls[0] -
-                      (fys[1] + fys[0]) * P.hrcf;
-    s.fzd[1] = STRICT ? qdiv(jf, cf) : jf * u.inv_cf;
-    s.fzd[0] = -s.fzd[1];
-    const double jr = fyg[3] * s.rStat[3] + fys[3] * ls[3] + fyg[2] * s.rStat[2] + fys[2] * ls[2] -
-                      (fys[3] + fys[2]) * P.hrcr;
-    s.fzd[3] = STRICT ? qdiv(jr, cr) : jr * u.inv_cr;
-    s.fzd[2] = -s.fzd[3];
-    double mx[4], my[4];
-    mx[0] = fys[0] * P.hrcf;
-    mx[1] = fys[1] * P.hrcf;
-    mx[2] = fys[2] * P.hrcr;
-    mx[3] = fys[3] * P.hrcr;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) my[i] = -(fxg[i] * s.rStat[i] + fxs[i] * ls[i]);
-    // computeVehicleRHS (dof24.cpp:1682-1733); the strut moments mz are zero
-    const double udot = s.wz * s.v - s.wy * s.w + k.inv_m * (fxs[0] + fxs[1] + fxs[2] + fxs[3]) + kG * sth;
-    const double vdot = s.wx * s.w - s.wz * s.u + k.inv_m * (fys[0] + fys[1] + fys[2] + fys[3]) - kG * sph * cth;
-    const double wdot =
-        s.wy * s.u - s.wx * s.v +
-        k.inv_m * (fzs[0] + fzs[1] + fzs[2] + fzs[3] + s.fzd[0] + s.fzd[1] + s.fzd[2] + s.fzd[3]) -
-        kG * cph * cth;
-    const double wxdot =
-        k.inv_jx * ((mx[0] + mx[1] + mx[2] + mx[3]) + (fzs[0] - fzs[1]) * k.half_cf + (fzs[2] - fzs[3]) * k.half_cr);
-    const double wydot = k.inv_jy * ((my[0] + my[1] + my[2] + my[3]) + (fzs[2] + fzs[3]) * b - (fzs[0] + fzs[1]) * a);
-    const double wzdot = k.inv_jz * (0.0 + (fys[0] + fys[1]) * a - (fys[2] + fys[3]) * b +
-                                     (-fxs[0] + fxs[1]) * k.half_cf + (-fxs[2] + fxs[3]) * k.half_cr);
-    // Half-implicit update (dof24_halfImplicit.cpp:214-281)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
+        const double dxt = s.vsz[i] - (cth * (s.wu[i] * cph + vu[i] * sph) - uu[i] * sth);
+        if (FULL) x->dxt[i] = dxt;
         if (!NR) {
             s.xe[i] = s.xe[i] + s.xedot[i] * dt;
             s.ye[i] = s.ye[i] + s.yedot[i] * dt;
         }
-        s.xt[i] = s.xt[i] + dxt[i] * dt;
-        s.omega[i] = s.omega[i] + dOm[i] * dt;
+        s.xt[i] = s.xt[i] + dxt * dt;
     }
+    // computePowertrainRHS (dof24.cpp:1338-1509)
+    double dOm[4], eng[4], dcrank;
+    powertrain4<STRICT>(P, u, k.tqs, k.maxBrake, throttle, braking, s.u, s.crank, s.gear, s.omega, fx, s.rStat, My, dOm, eng,
+                dcrank);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        s.wu[i] = s.wu[i] + dwu[i] * dt;
-        s.xs[i] = s.xs[i] + s.dxs[i] * dt;
+        if (FULL) {
+            x->dOmega[i] = dOm[i]; x->engTor[i] = eng[i]; x->fz[i] = fz[i]; x->My[i] = My[i];
+        }
+        s.omega[i] = s.omega[i] + dOm[i] * dt;
     }
     if (P.tcbool) s.crank = s.crank + dcrank * dt;
+    // Per strut, front axle then rear: tireToVehTransform (dof24.cpp:459-528), computeSusRHS
+    // (:1182-1229: last step's fzd and dxs, then the new dxs) and computeForcesThroughSus (:599-680;
+    // the rear-right lateral force has no cos(theta) factor in the reference (:639) and keeps that
+    // form here), then the axle's jacking force.  The sums of computeVehicleRHS (dof24.cpp:1682-1733;
+    // the strut moments mz are zero) accumulate as the struts come, in the reference's order.
+    const double sph_f = STRICT ? sin(s.phi) : sph;
+    double sfxs = 0, sfys = 0, sfzs = 0, smx = 0, smy = 0;
+    double tF = 0, sF = 0, tA = 0, tC = 0, tR = 0, sR = 0, tB = 0, tD = 0;
+#pragma unroll
+    for (int ax = 0; ax < 2; ++ax) {
+        const double mu = ax == 0 ? P.muf : P.mur;
+        const double hrc = ax == 0 ? P.hrcf : P.hrcr;
+        double fyg_a[2], fxs_a[2], fys_a[2], fzs_a[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int i = 2 * ax + j;
+            double fxi = fx[i], fyi = fy[i];
+            if (ax == 0) {
+                fxi = fx[i] * cd - fy[i] * sd;
+                fyi = fx[i] * sd + fy[i] * cd;
+            }
+            const double fxg = fxi * cth - fz[i] * sth;
+            const double fyg = fxi * sph * sth + fyi * cph + fz[i] * sph * cth;
+            const double fzg = fxi * sth * cph - fyi * sph + fz[i] * cph * cth;
+            const double dwu = k.inv_mu[ax] * (cph * (cth * (fz[i] - k.mu_g[ax]) + sth * fxi) - sph * fyi - s.fzd[i] -
+                                               s.xs[i] * P.ks - s.dxs[i] * P.bs - mu * (vu[i] * s.wx - uu[i] * s.wy));
+            double dus, dvs, ws;
+            if (i == 0) {
+                dus = (-cf * s.wzdot) / 2 + s.udot;
+                dvs = a * s.wzdot + s.vdot;
+                ws = (cf * s.wx / 2.) - (a * s.wy) + s.w;
+            } else if (i == 1) {
+                dus = (cf * s.wzdot) / 2 + s.udot;
+                dvs = a * s.wzdot + s.vdot;
+                ws = (-cf * s.wx / 2.) - (a * s.wy) + s.w;
+            } else if (i == 2) {
+                dus = (-cr * s.wzdot) / 2 + s.udot;
+                dvs = -b * s.wzdot + s.vdot;
+                ws = (cr * s.wx / 2.) + b * s.wy + s.w;
+            } else {
+                dus = (cr * s.wzdot) / 2 + s.udot;
+                dvs = (-b * s.wzdot) + s.vdot;
+                ws = (-cr * s.wx / 2.) + b * s.wy + s.w;
+            }
+            const double duu = dus - (-s.dxs[i] * s.wy + ls[i] * s.wydot);
+            const double dvu = dvs + (-s.dxs[i] * s.wx + ls[i] * s.wxdot);
+            s.dxs[i] = -ws + s.wu[i];
+            const double fxs = fxg + k.mu_g[ax] * sth - mu * duu + mu * s.wz * vu[i] - mu * s.wy * s.wu[i];
+            const double fys = i < 3 ? fyg - k.mu_g[ax] * sph_f * cth - mu * dvu + mu * s.wx * s.wu[i] - mu * s.wz * uu[i]
+                                     : fyg - k.mu_g[ax] * sph_f - mu * dvu + mu * s.wx * s.wu[i] - mu * s.wz * uu[i];
+            const double fzs = s.xs[i] * P.ks + s.dxs[i] * P.bs;
+            const double mx = fys * hrc;
+            const double my = -(fxg * s.rStat[i] + fxs * ls[i]);
+            s.wu[i] = s.wu[i] + dwu * dt;
+            s.xs[i] = s.xs[i] + s.dxs[i] * dt;
+            sfxs = i == 0 ? fxs : sfxs + fxs;
+            sfys = i == 0 ? fys : sfys + fys;
+            sfzs = i == 0 ? fzs : sfzs + fzs;
+            smx = i == 0 ? mx : smx + mx;
+            smy = i == 0 ? my : smy + my;
+            fyg_a[j] = fyg;
+            fxs_a[j] = fxs;
+            fys_a[j] = fys;
+            fzs_a[j] = fzs;
+            if (FULL) {
+                x->fx[i] = fxi; x->fy[i] = fyi; x->fxg[i] = fxg; x->fyg[i] = fyg; x->fzg[i] = fzg;
+                x->ws[i] = ws; x->dus[i] = dus; x->dvs[i] = dvs; x->duu[i] = duu; x->dvu[i] = dvu; x->dwu[i] = dwu;
+                x->fxs[i] = fxs; x->fys[i] = fys; x->fzs[i] = fzs; x->mx[i] = mx; x->my[i] = my;
+            }
+        }
+        // the axle's jacking force, from this step's static radii
+        const int i0 = 2 * ax, i1 = 2 * ax + 1;
+        const double jk = fyg_a[1] * s.rStat[i1] + fys_a[1] * ls[i1] + fyg_a[0] * s.rStat[i0] + fys_a[0] * ls[i0] -
+                          (fys_a[1] + fys_a[0]) * hrc;
+        if (ax == 0) {
+            s.fzd[1] = STRICT ? qdiv(jk, cf) : jk * u.inv_cf;
+            s.fzd[0] = -s.fzd[1];
+            tF = (fzs_a[0] - fzs_a[1]) * k.half_cf;
+            sF = fzs_a[0] + fzs_a[1];
+            tA = (fys_a[0] + fys_a[1]) * a;
+            tC = (-fxs_a[0] + fxs_a[1]) * k.half_cf;
+        } else {
+            s.fzd[3] = STRICT ? qdiv(jk, cr) : jk * u.inv_cr;
+            s.fzd[2] = -s.fzd[3];
+            tR = (fzs_a[0] - fzs_a[1]) * k.half_cr;
+            sR = fzs_a[0] + fzs_a[1];
+            tB = (fys_a[0] + fys_a[1]) * b;
+            tD = (-fxs_a[0] + fxs_a[1]) * k.half_cr;
+        }
+    }
+    const double udot = s.wz * s.v - s.wy * s.w + k.inv_m * sfxs + kG * sth;
+    const double vdot = s.wx * s.w - s.wz * s.u + k.inv_m * sfys - kG * sph * cth;
+    const double wdot = s.wy * s.u - s.wx * s.v + k.inv_m * (sfzs + s.fzd[0] + s.fzd[1] + s.fzd[2] + s.fzd[3]) -
+                        kG * cph * cth;
+    const double wxdot = k.inv_jx * (smx + tF + tR);
+    const double wydot = k.inv_jy * (smy + sR * b - sF * a);
+    const double wzdot = k.inv_jz * (0.0 + tA - tB + tC + tD);
     s.u = s.u + udot * dt;
     s.v = s.v + vdot * dt;
     s.w = s.w + wdot * dt;
@@ -257,28 +313,14 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     s.theta = s.theta + dtheta * dt;
     s.psi = s.psi + dpsi * dt;
     s.phi = s.phi + dphi * dt;
-    double sp, cp;
-    if (STRICT) sincos(s.psi, &sp, &cp);
-    else sincos_fast(s.psi, sp, cp);
-    s.x = s.x + (s.u * cp - s.v * sp) * dt;
-    s.y = s.y + (s.u * sp + s.v * cp) * dt;
     s.udot = udot;
     s.vdot = vdot;
     s.wdot = wdot;
     s.wxdot = wxdot;
     s.wydot = wydot;
     s.wzdot = wzdot;
+    if (POS) position24<STRICT>(s, dt);
     if (FULL) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            x->dOmega[i] = dOm[i]; x->dxt[i] = dxt[i]; x->fx[i] = fx[i]; x->fy[i] = fy[i]; x->fz[i] = fz[i];
-            x->fxg[i] = fxg[i]; x->fyg[i] = fyg[i]; x->fzg[i] = fzg[i]; x->vgx[i] = vgx[i]; x->vgy[i] = vgy[i];
-            x->vsx[i] = vsx[i]; x->vsy[i] = vsy[i]; x->My[i] = My[i]; x->engTor[i] = eng[i];
-            x->ls[i] = ls[i]; x->us[i] = us[i]; x->vs[i] = vs[i]; x->ws[i] = ws[i]; x->dus[i] = dus[i];
-            x->dvs[i] = dvs[i]; x->uu[i] = uu[i]; x->vu[i] = vu[i]; x->duu[i] = duu[i]; x->dvu[i] = dvu[i];
-            x->dwu[i] = dwu[i]; x->fxs[i] = fxs[i]; x->fys[i] = fys[i]; x->fzs[i] = fzs[i]; x->mx[i] = mx[i];
-            x->my[i] = my[i];
-        }
         x->dtheta = dtheta;
         x->dphi = dphi;
         x->dpsi = dpsi;

@@ -24,6 +24,18 @@ constexpr double kHalfSqrt2 = 0.70710678118654757;              // sqrt(2.) / 2.
 // the constants where it uses them instead of holding them in registers for the whole loop.
 #define LFDM_RELOAD_MEMORY() asm volatile("" ::: "memory")
 
+// Makes v opaque to the compiler (no instruction is emitted; the value is unchanged): expressions
+// computed from it afterwards are not merged with the same expressions computed from it before, so
+// code that recomputes a cheap value near a late use does not get the early result held live in
+// registers in between instead.
+LFDM_HD void opaque(double& v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+v"(v));
+#else
+    (void)v;
+#endif
+}
+
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Quotient and square root of the step code.  On the device: Newton-Raphson refinement of the
@@ -180,23 +192,119 @@ LFDM_HD double cos_poly(double z) {
     return __builtin_fma(z, p, 1.0);
 }
 
+// Word k of the bits of 2/pi, 32 per word, most significant first: word k >= 2 holds bits
+// 32(k-2)+1 .. 32(k-1) after the binary point, words 0 and 1 (the bits at and above it) are zero.
+// 37 words of 2/pi, computed exactly in integer arithmetic (Machin's formula for pi; checked by
+// tests/test_strict_host.py).
+LFDM_HD unsigned int two_over_pi_word(int k) {
+    static constexpr unsigned int w[39] = {
+        0x00000000u, 0x00000000u, 0xa2f9836eu, 0x4e441529u, 0xfc2757d1u, 0xf534ddc0u,
+        0xdb629599u, 0x3c439041u, 0xfe5163abu, 0xdebbc561u, 0xb7246e3au, 0x424dd2e0u,
+        0x06492eeau, 0x09d1921cu, 0xfe1deb1cu, 0xb129a73eu, 0xe88235f5u, 0x2ebb4484u,
+        0xe99c7026u, 0xb45f7e41u, 0x3991d639u, 0x835339f4u, 0x9c845f8bu, 0xbdf9283bu,
+        0x1ff897ffu, 0xde05980fu, 0xef2f118bu, 0x5a0a6d1fu, 0x6d367ecfu, 0x27cb09b7u,
+        0x4f463f66u, 0x9e5fea2du, 0x7527bac7u, 0xebe5f17bu, 0x3d0739f7u, 0x8a5292eau,
+        0x6bfb5fb1u, 0x1f8d5d08u, 0x56033046u,
+    };
+    return w[k];
+}
+
+// Payne-Hanek reduction (M. Payne, R. Hanek, "Radian reduction for trigonometric functions",
+// SIGNUM Newsletter 18(1), 1983) of a finite x >= 2^20: x 2/pi = n + f with f in [-1/2, 1/2);
+// returns n mod 4 and r = f pi/2 as rh + rl.  With x = M 2^e (M the 53-bit integer significand,
+// e >= -32), the words of 2/pi whose products with M are multiples of 4 are skipped; the next seven
+// (224 bits) are multiplied by M exactly in 32-bit limbs, and the product's 128 bits from two above
+// the binary point down give n mod 4 and f to 126 bits (the words left out weigh below 2^-170 in
+// x 2/pi, far below the 2^-62 of the closest double to a multiple of pi/2).  f goes to
+// double-double and is multiplied by pi/2 in two parts.  Integer arithmetic only up to there, in
+// about thirty registers.
+LFDM_HD int reduce_pio2_large(double x, double& rh, double& rl) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const int e = (int)((b >> 52) & 0x7ff) - 1075;
+    const unsigned long long m = (b & 0xfffffffffffffull) | 0x10000000000000ull;
+    const int k0 = (e + 62) >> 5;       // first word whose product with M is not a multiple of 4
+    const int rho = e + 62 - 32 * k0;   // the binary point of the product is at bit 222 - rho
+    const unsigned int m0 = (unsigned int)m, m1 = (unsigned int)(m >> 32);
+    unsigned int L[7];
+    unsigned long long c = 0;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const unsigned long long t = (unsigned long long)two_over_pi_word(k0 + 6 - j) * m0 + c;
+        L[j] = (unsigned int)t;
+        c = t >> 32;
+    }
+    c = 0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const unsigned long long t = (unsigned long long)two_over_pi_word(k0 + 6 - j) * m1 + L[j + 1] + c;
+        L[j + 1] = (unsigned int)t;
+        c = t >> 32;
+    }
+    // bits 96 - rho .. 223 - rho of the product: two integer bits, then the fraction
+    const unsigned long long p2 = (unsigned long long)L[6] << 32 | L[5];
+    const unsigned long long p1 = (unsigned long long)L[4] << 32 | L[3];
+    const unsigned long long p0 = (unsigned long long)L[2] << 32;
+    const unsigned long long qh = (p2 << rho) | ((p1 >> 1) >> (63 - rho));
+    const unsigned long long ql = (p1 << rho) | ((p0 >> 1) >> (63 - rho));
+    int n = (int)(qh >> 62);
+    unsigned long long fh = (qh << 2) | (ql >> 62), fl = ql << 2;  // f in [0, 1) as fh:fl / 2^128
+    const bool up = (fh >> 63) != 0;                                 // f >= 1/2: n + 1, f - 1
+    if (up) {
+        n += 1;
+        fl = ~fl + 1;
+        fh = ~fh + (fl == 0 ? 1 : 0);
+    }
+    // |f| = fh 2^-64 + fl 2^-128 (fh < 2^63) to double-double, then times pi/2
+    const double dh = (double)fh;
+    const long long eh = (long long)(fh - (unsigned long long)dh);
+    const double f0 = dh * 0x1p-64;
+    const double f1 = ((double)eh + (double)fl * 0x1p-64) * 0x1p-64;
+    const double s = f0 + f1;
+    const double t = f1 - (s - f0);
+    const double ph = s * 0x1.921fb54442d18p+0;
+    const double pl = __builtin_fma(s, 0x1.921fb54442d18p+0, -ph) + (s * 0x1.1a62633145c07p-54 + t * 0x1.921fb54442d18p+0);
+    rh = ph + pl;
+    rl = pl - (rh - ph);
+    if (up) {
+        rh = -rh;
+        rl = -rl;
+    }
+    return n & 3;
+}
+
 // sin(a) and cos(a) of the fast form: Cody-Waite reduction a = k pi/2 + r with pi/2 in two parts
 // (hi, the double nearest pi/2, and mid, the double nearest pi/2 - hi; the first fused step is exact
 // and the neglected third part contributes below |k| 1e-32), the polynomials above, and
 // the quadrant's swap and signs.  Within about one ulp of the correctly rounded results for |a| below
-// 2^20 pi/2 (~1.6e6); larger arguments go to the device libm.  About half the instructions of the
-// libm's sincos, which reduces in double-double arithmetic.
+// 2^20 pi/2 (~1.6e6); larger finite arguments are reduced by reduce_pio2_large and the reduced
+// argument's low part enters to first order (sin(r + rl) = sin r + rl cos r).  Infinities and NaN
+// give NaN.  About half the instructions of the libm's sincos, which reduces in double-double
+// arithmetic, and no call into it: the libm's reduction of huge arguments, inlined into every step,
+// held more registers than the step's own code.
 LFDM_HD void sincos_fast(double a, double& s, double& c) {
-    if (!(fabs(a) < 0x1.921fb54442d18p+20)) {
-        sincos(a, &s, &c);
-        return;
+    double sr, cr;
+    int q;
+    if (fabs(a) < 0x1.921fb54442d18p+20) {
+        const double k = __builtin_rint(a * 0x1.45f306dc9c883p-1);  // a * 2/pi
+        double r = __builtin_fma(-k, 0x1.921fb54442d18p+0, a);
+        r = __builtin_fma(-k, 0x1.1a62633145c07p-54, r);
+        const double z = r * r;
+        sr = sin_poly(r, z);
+        cr = cos_poly(z);
+        q = (int)k;
+    } else {
+        double rh, rl;
+        q = reduce_pio2_large(fabs(a), rh, rl);
+        const double z = rh * rh;
+        const double s1 = sin_poly(rh, z), c1 = cos_poly(z);
+        sr = __builtin_fma(rl, c1, s1);
+        cr = __builtin_fma(-rl, s1, c1);
+        if (a < 0) {  // -a = q pi/2 + r: a = (-q) pi/2 - r
+            sr = -sr;
+            q = -q;
+        }
+        if (!(fabs(a) <= 0x1.fffffffffffffp+1023)) sr = cr = a - a;  // inf, NaN
     }
-    const double k = __builtin_rint(a * 0x1.45f306dc9c883p-1);  // a * 2/pi
-    double r = __builtin_fma(-k, 0x1.921fb54442d18p+0, a);
-    r = __builtin_fma(-k, 0x1.1a62633145c07p-54, r);
-    const double z = r * r;
-    const double sr = sin_poly(r, z), cr = cos_poly(z);
-    const int q = (int)k;
     const bool swap = (q & 1) != 0;
     const double s0 = swap ? cr : sr, c0 = swap ? sr : cr;
     s = (q & 2) ? -s0 : s0;

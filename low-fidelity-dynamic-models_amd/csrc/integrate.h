@@ -125,11 +125,13 @@ struct M18 {
     static LFDM_HD void kinit(const PSet& P, const double* o, size_t S, K18& k) {
         k18_init(P, o[0], o[S], o[2 * S], o[3 * S], o[4 * S], o[5 * S], o[6 * S], k);
     }
-    template <bool NR, bool FULL, bool STRICT>
+    template <bool NR, bool FULL, bool STRICT, bool POS = true>
     static LFDM_HD void step(const PSet& P, const Derived& u, const K18& k, S18& s, const Ctl& c,
                                                 double dt, X18* x) {
-        step18<NR, FULL, false, STRICT>(P, u, k, s, c.s, c.th, c.b, dt, x);
+        step18<NR, FULL, false, STRICT, POS>(P, u, k, s, c.s, c.th, c.b, dt, x);
     }
+    template <bool STRICT>
+    static LFDM_HD void position(S18& s, double dt) { position18<STRICT>(s, dt); }
     static __device__ __forceinline__ void emit(double* o, size_t nu, double t, const S18& s) {
         o[0 * nu] = t; o[1 * nu] = s.x; o[2 * nu] = s.y; o[3 * nu] = s.u; o[4 * nu] = s.v;
         o[5 * nu] = s.phi; o[6 * nu] = s.psi; o[7 * nu] = s.wx; o[8 * nu] = s.wz;
@@ -198,11 +200,13 @@ struct M11 {
     static LFDM_HD void kinit(const PSet& P, const double* o, size_t S, K11& k) {
         k11_init(P, o[0], o[S], o[3 * S], o[4 * S], o[5 * S], o[6 * S], k);
     }
-    template <bool NR, bool FULL, bool STRICT>
+    template <bool NR, bool FULL, bool STRICT, bool POS = true>
     static LFDM_HD void step(const PSet& P, const Derived& u, const K11& k, S11& s, const Ctl& c,
                                                 double dt, X11* x) {
-        step11<NR, FULL, STRICT>(P, u, k, s, c.s, c.th, c.b, dt, x);
+        step11<NR, FULL, STRICT, POS>(P, u, k, s, c.s, c.th, c.b, dt, x);
     }
+    template <bool STRICT>
+    static LFDM_HD void position(S11& s, double dt) { position11<STRICT>(s, dt); }
     static __device__ __forceinline__ void emit(double* o, size_t nu, double t, const S11& s) {
         o[0 * nu] = t; o[1 * nu] = s.x; o[2 * nu] = s.y; o[3 * nu] = s.u; o[4 * nu] = s.v;
         o[5 * nu] = s.psi; o[6 * nu] = s.wz; o[7 * nu] = s.omega[0]; o[8 * nu] = s.omega[1];
@@ -285,11 +289,13 @@ struct M24 {
     static LFDM_HD void kinit(const PSet& P, const double* o, size_t S, K24& k) {
         k24_init(P, o[0], o[S], o[2 * S], o[3 * S], o[4 * S], o[5 * S], o[6 * S], k);
     }
-    template <bool NR, bool FULL, bool STRICT>
+    template <bool NR, bool FULL, bool STRICT, bool POS = true>
     static LFDM_HD void step(const PSet& P, const Derived& u, const K24& k, S24& s, const Ctl& c,
                                                 double dt, X24* x) {
-        step24<NR, FULL, STRICT>(P, u, k, s, c.s, c.th, c.b, dt, x);
+        step24<NR, FULL, STRICT, POS>(P, u, k, s, c.s, c.th, c.b, dt, x);
     }
+    template <bool STRICT>
+    static LFDM_HD void position(S24& s, double dt) { position24<STRICT>(s, dt); }
     static __device__ __forceinline__ void emit(double* o, size_t nu, double t, const S24& s) {
         o[0 * nu] = t; o[1 * nu] = s.x; o[2 * nu] = s.y; o[3 * nu] = s.u; o[4 * nu] = s.v;
         o[5 * nu] = s.phi; o[6 * nu] = s.psi; o[7 * nu] = s.theta; o[8 * nu] = s.wx; o[9 * nu] = s.wz;
@@ -341,10 +347,11 @@ struct M24 {
 
 // Occupancy of the integration kernels (waves per SIMD).  Each model has a default, M::kWaves, the
 // most waves per SIMD it is built for: on large fleets more resident waves hide more FP64 latency
-// than the register spills of a smaller budget cost, up to four for the 11-DOF step (292 B of spills
-// per lane, +4.5 % over three) and three for the 18-DOF step (436 B, +10 % over two; four would spill
-// 596 B and run 17 % slower than three); the 24-DOF step would spill over a kilobyte per lane at two
-// waves and runs 1.2x faster at one.  A fleet that fills at most one wavefront per SIMD gains nothing
+// than the register spills of a smaller budget cost, up to four for the 11-DOF step (five run 2 %
+// slower), three for the 18-DOF step (four spill twice as much and run 12 % slower) and two for the
+// 24-DOF step (1.37x faster than one; three would spill over a kilobyte per lane).  The spills are
+// counted for the step loop alone: the last step, which also returns the full record, holds more
+// registers, but runs once.  A fleet that fills at most one wavefront per SIMD gains nothing
 // from the other waves' register budget, so every kernel also exists at every lower occupancy;
 // Launch::waves picks (0: the model's default).  STRICT instantiations evaluate the step with the
 // reference's own quotients and transcendentals (Launch::strict; built at the model's default
@@ -386,28 +393,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
     // them; the rest after the last step
     int oi = 0;
     int next_out = L.n_out > 0 ? L.out_steps[0] : -1;
-    // all steps but the lane's last keep only the carried state ...
+    // all steps but the lane's last keep only the carried state ...  Each step's position update
+    // (x, y and, in the 11- and 18-DOF models, psi; nothing else in a step reads them) is applied at
+    // the start of the next step, or before an output row or the last step reads the state: the
+    // same operations on the same values, without the yaw angle's sine and cosine, which have a
+    // branch for huge angles, splitting the step's code where it holds the most registers
+    bool pend = false;  // this lane's last step left its position update
+    int clean = 0;      // a lane stepping at kk != clean has stepped at kk - 1 with no output row since
     for (int kk = 0; kk < wsteps - 1; ++kk) {
         if constexpr (M::kKInLds) LFDM_RELOAD_MEMORY();
         if (kk < my - 1) {
+            if (kk != clean) M::template position<STRICT>(s, dt);  // the same branch on every lane
             const Ctl c = controls_at<MODE>(L, P, cs, uv, kk, t, f);
-            M::template step<NR, false, STRICT>(P, u, k, s, c, dt, nullptr);
+            M::template step<NR, false, STRICT, false>(P, u, k, s, c, dt, nullptr);
+            pend = true;
             t = t + dt;
         }
         if (kk + 1 == next_out) {
+            if (pend) M::template position<STRICT>(s, dt);
+            pend = false;
+            clean = kk + 1;
             if (kk + 1 < my && user >= 0)
                 M::emit(L.out + (size_t)oi * M::kCh * L.n_user + user, (size_t)L.n_user, t, s);
             ++oi;
             next_out = oi < L.n_out ? L.out_steps[oi] : -1;
         }
     }
+    if (pend) M::template position<STRICT>(s, dt);
     // ... the last one also returns the right-hand-side quantities the full record holds (a lane
     // whose horizon comes earlier than its wavefront's takes it at its own step and time)
     if (my > 0) {
         typename M::X x;
         CtrlState<MODE> cl;
         const Ctl c = controls_at<MODE>(L, P, my == wsteps ? cs : cl, uv, my - 1, t, f);
-        M::template step<NR, true, STRICT>(P, u, k, s, c, dt, &x);
+        M::template step<NR, true, STRICT, false>(P, u, k, s, c, dt, &x);
+        M::template position<STRICT>(s, dt);
         t = t + dt;
         M::template store<NR>(L.state, S, slot, P, s, x);
     }

@@ -21,7 +21,7 @@ __device__ __forceinline__ const PSet& pset_at(const PSet* sets, int i) {
 
 // Most waves per SIMD each model's integration kernels are built for (each also exists at every lower
 // occupancy; the maximum is the default on fleets of more than one wavefront per SIMD).
-constexpr int kMaxWaves11 = 4, kMaxWaves18 = 3, kMaxWaves24 = 1;
+constexpr int kMaxWaves11 = 4, kMaxWaves18 = 3, kMaxWaves24 = 2;
 
 struct Launch {
     const PSet* sets;       // parameter sets

@@ -2,8 +2,9 @@
 // the compiler's IEEE `/` and sqrt on the device and against the host's correctly rounded results, and
 // of the fast form's faithful quotient, reciprocal and square root (fdiv, frcp, fsqrt): each within one
 // ulp of the correctly rounded result; and of the fast form's sine and cosine (sincos_fast over angles
-// up to 1e5 rad, sin_2pi_unit over [0, 1]): within two ulps of the correctly rounded results (host long
-// double sinl / cosl, rounded to double); and of fsqrt_rcp (square root and reciprocal square root from
+// up to 1e5 rad and over huge ones, 2^20.7 to 2^1024 rad, which take its Payne-Hanek reduction;
+// sin_2pi_unit over [0, 1]): within two ulps of the correctly rounded results (host long double
+// sinl / cosl, rounded to double), NaN for infinite and NaN angles; and of fsqrt_rcp (square root and reciprocal square root from
 // one v_rsq_f64): each within one ulp.
 //
 // Operands: random signs, mantissas and exponents over the range the helpers are specified for
@@ -137,14 +138,20 @@ int main(int argc, char** argv) {
     std::vector<double> hf(5 * (size_t)n);
     CHECK(hipMemcpy(hf.data(), fo, 5 * bytes, hipMemcpyDeviceToHost));
     CHECK(hipFree(fo));
-    // angles: steering-sized (|a| <= 0.6), a few turns (|a| <= 20), yaw after long runs (|a| <= 1e5);
+    // angles: steering-sized (|a| <= 0.6), a few turns (|a| <= 20), yaw after long runs (|a| <= 1e5),
+    // huge (|a| = 2^U(20.7, 1023.99), the large-argument reduction); the last three are inf, -inf, NaN;
     // sine-step arguments v in [0, 1]
     std::vector<double> ang(n), unit(n);
-    std::uniform_real_distribution<double> u06(-0.6, 0.6), u20(-20.0, 20.0), u1e5(-1e5, 1e5), u01(0.0, 1.0);
+    std::uniform_real_distribution<double> u06(-0.6, 0.6), u20(-20.0, 20.0), u1e5(-1e5, 1e5), u01(0.0, 1.0),
+        uexp(20.7, 1023.99);
     for (int i = 0; i < n; ++i) {
-        ang[i] = i % 3 == 0 ? u06(rng) : i % 3 == 1 ? u20(rng) : u1e5(rng);
+        const int c = i % 4;
+        ang[i] = c == 0 ? u06(rng) : c == 1 ? u20(rng) : c == 2 ? u1e5(rng) : (i & 4 ? -1 : 1) * std::exp2(uexp(rng));
         unit[i] = i % 1024 == 0 ? (double)(i / 1024 % 5) * 0.25 : u01(rng);  // include 0, 1/4, 1/2, 3/4, 1
     }
+    ang[n - 3] = HUGE_VAL;
+    ang[n - 2] = -HUGE_VAL;
+    ang[n - 1] = NAN;
     double *dang, *dunit, *to;
     CHECK(hipMalloc(&dang, bytes));
     CHECK(hipMalloc(&dunit, bytes));
@@ -216,7 +223,12 @@ int main(int argc, char** argv) {
     // fast sine and cosine against the host's long double results rounded to double
     const long double two_pi = 6.283185307179586476925286766559005768L;
     long long tmax[3] = {0, 0, 0};
+    long long nonfinite_bad = 0;
     for (int i = 0; i < n; ++i) {
+        if (!std::isfinite(ang[i])) {
+            nonfinite_bad += !std::isnan(ht[i]) || !std::isnan(ht[(size_t)n + i]);
+            continue;
+        }
         // sin(2 pi v) = sin(2 pi t) for the exactly reduced t of sin_2pi_unit (v - rint(v), then +-1/2 - r),
         // whose long double product with 2 pi has no cancellation near the zeros
         const double r = unit[i] - std::rint(unit[i]);
@@ -233,8 +245,9 @@ int main(int argc, char** argv) {
             }
         }
     }
-    std::printf("sincos_fast: sine at most %lld ulp, cosine at most %lld ulp from the correctly rounded values\n",
-                tmax[0], tmax[1]);
+    std::printf("sincos_fast: sine at most %lld ulp, cosine at most %lld ulp from the correctly rounded values "
+                "(angles up to 2^1024); %lld of inf, -inf, NaN not NaN\n",
+                tmax[0], tmax[1], nonfinite_bad);
     std::printf("sin_2pi_unit: at most %lld ulp from the correctly rounded value\n", tmax[2]);
     // sqrt and reciprocal square root of fsqrt_rcp (the reciprocal for x > 0; long double reference)
     long long smax[2] = {0, 0};
@@ -250,7 +263,7 @@ int main(int argc, char** argv) {
     }
     std::printf("fsqrt_rcp: square root at most %lld ulp, reciprocal at most %lld ulp from the correctly rounded values\n",
                 smax[0], smax[1]);
-    const bool ok = qd == 0 && sd == 0 && fmax_[0] <= 1 && fmax_[1] <= 1 && fmax_[2] <= 1 && tmax[0] <= 2 &&
+    const bool ok = qd == 0 && sd == 0 && nonfinite_bad == 0 && fmax_[0] <= 1 && fmax_[1] <= 1 && fmax_[2] <= 1 && tmax[0] <= 2 &&
                     tmax[1] <= 2 && tmax[2] <= 2 && smax[0] <= 1 && smax[1] <= 1;
     std::printf("%s\n", ok ? "ALL OK" : "MISMATCH");
     return ok ? 0 : 1;

@@ -98,4 +98,7 @@ void host_fast_trig(const double* a, const double* v, int n, double* out) {
     }
 }
 
+// Word k of the 2/pi table of the large-argument reduction.
+unsigned int host_two_over_pi_word(int k) { return dev::two_over_pi_word(k); }
+
 }  // extern "C"

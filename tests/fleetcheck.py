@@ -131,7 +131,7 @@ def reference(O, case, nsteps, every):
     return Reference(ref, upto, spread, nf_flip)
 
 
-def _judge(err, spread, gpu_nf, ref_nf, nf_flip, mult):
+def _assess(err, spread, gpu_nf, ref_nf, nf_flip, mult):
     stable = spread <= STABLE
     with np.errstate(over="ignore", invalid="ignore"):
         bound = np.where(stable, TOL, np.maximum(TOL, mult * spread))
@@ -152,7 +152,7 @@ def verdict(gpu, R, mult=None):
     gpu_nonfinite = gup < gpu.shape[0]
     ref_nonfinite = R.upto < ref.shape[0]
     err = normwise(gpu, ref, np.minimum(R.upto, gup))
-    stable, ok_v, nf_ok, fr = _judge(err, spread, gpu_nonfinite, ref_nonfinite, R.nf_flip, mult)
+    stable, ok_v, nf_ok, fr = _assess(err, spread, gpu_nonfinite, ref_nonfinite, R.nf_flip, mult)
     fin = np.isfinite(ref).all(axis=2) & np.isfinite(gpu).all(axis=2)
     return dict(
         n=n,
@@ -277,7 +277,7 @@ def final_verdict(gpu, R, mult=None):
     gpu_nf = ~np.isfinite(gpu).all(axis=-1)
     both = ~(ref_nf | gpu_nf)
     err = np.where(both, final_err(np.where(both[:, None], gpu, 0), np.where(both[:, None], ref, 0), R.den), 0.0)
-    stable, ok_v, nf_ok, fr = _judge(err, spread, gpu_nf, ref_nf, R.nf_flip, mult)
+    stable, ok_v, nf_ok, fr = _assess(err, spread, gpu_nf, ref_nf, R.nf_flip, mult)
     return dict(
         n=ref.shape[0],
         ok=bool(ok_v.all()),

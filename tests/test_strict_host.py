@@ -155,3 +155,49 @@ def test_fast_sines_and_cosines_on_the_host():
     o = np.empty(3)
     L.host_fast_trig(nan.ctypes.data, nan.ctypes.data, 1, o.ctypes.data)
     assert np.isnan(o).all()
+
+
+def test_fast_sines_and_cosines_of_huge_angles_on_the_host():
+    """sincos_fast beyond its Cody-Waite range (|a| >= 2^20 pi/2) reduces by Payne-Hanek with a 2/pi
+    table: against mpmath at 2,000 bits on log-uniform angles up to 2^1024 and on the double closest to
+    a multiple of pi/2 (6381956970095103 2^797, where cos is 4.7e-19), within one ulp; NaN for inf, NaN;
+    and the table's 37 words are the bits of 2/pi (Machin's formula in integer arithmetic)."""
+    mpmath = pytest.importorskip("mpmath")
+    L = host_lib()
+    L.host_fast_trig.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    L.host_two_over_pi_word.restype = ctypes.c_uint
+    L.host_two_over_pi_word.argtypes = [ctypes.c_int]
+
+    def atan_inv(x, one):  # atan(1/x) * one
+        s, t, k, sign = 0, one // x, 1, 1
+        while t:
+            s += sign * (t // k)
+            t //= x * x
+            k += 2
+            sign = -sign
+        return s
+    prec = 1400
+    one = 1 << prec
+    pi = 16 * atan_inv(5, one) - 4 * atan_inv(239, one)
+    bits = (2 << (prec + 37 * 32)) // pi  # floor(2/pi 2^1184), exact but for the last few bits
+    words = [(bits >> (32 * (36 - k))) & 0xffffffff for k in range(37)]
+    assert [L.host_two_over_pi_word(k) for k in range(39)] == [0, 0] + words
+
+    rng = np.random.default_rng(5)
+    n = 3000
+    a = np.exp2(rng.uniform(20.66, 1023.99, n)) * rng.choice([-1.0, 1.0], n)
+    a = np.concatenate([a, [6381956970095103 * 2.0 ** 797, -6381956970095103 * 2.0 ** 797, 2.0 ** 1023,
+                            np.finfo(np.float64).max, float.fromhex("0x1.921fb54442d18p+20")]])
+    out = np.empty(3 * len(a))
+    L.host_fast_trig(a.ctypes.data, a.ctypes.data, len(a), out.ctypes.data)
+    s, c = out[:len(a)], out[len(a):2 * len(a)]
+    mpmath.mp.prec = 2000
+    rs = np.array([float(mpmath.sin(mpmath.mpf(float(x)))) for x in a])
+    rc = np.array([float(mpmath.cos(mpmath.mpf(float(x)))) for x in a])
+    us = np.abs(s - rs) / np.spacing(np.abs(rs))
+    uc = np.abs(c - rc) / np.spacing(np.abs(rc))
+    assert us.max() <= 1 and uc.max() <= 1, (us.max(), uc.max(), a[us.argmax()], a[uc.argmax()])
+    nf = np.array([np.inf, -np.inf, np.nan])
+    o = np.empty(9)
+    L.host_fast_trig(nf.ctypes.data, nf.ctypes.data, 3, o.ctypes.data)
+    assert np.isnan(o[:6]).all()
