@@ -46,8 +46,6 @@ LFDM_HD void k11_init(const PSet& P, double m, double jz, double dfy0Pn, double 
     k.mt = m + 2 * (muf + mur);
 }
 
-// One half-implicit step (rhsFun order, dof11_halfImplicit.cpp:560-605; update :172-235).
-// STRICT: the reference's own quotients and transcendentals throughout (see dev_tire.h).
 // The step's last part, the position update from the new velocities at the step's yaw angle, then
 // the yaw angle (dof11_halfImplicit.cpp:214-219).  Nothing else in a step reads x, y or psi: a step loop may
 // leave it out (POS = false) and apply it at the start of the next step or before the state is read,
@@ -62,6 +60,9 @@ LFDM_HD void position11(S11& s, double dt) {
     s.psi = s.psi + s.wz * dt;
 }
 
+// One half-implicit step (rhsFun order, dof11_halfImplicit.cpp:560-605; update :172-235).
+// STRICT: the reference's own quotients and transcendentals throughout (see dev_tire.h).
+// POS = false leaves out the closing position update (position11).
 template <bool NR, bool FULL, bool STRICT = false, bool POS = true>
 LFDM_HD void step11(const PSet& P, const Derived& u, const K11& k, S11& s, double steering,
                                        double throttle, double braking, double dt, X11* x) {

@@ -99,14 +99,6 @@ LFDM_HD void tire_tmeasy_nr(const TireP& t, const Derived& u, const K18& k, doub
     tmeasy_nr_core<STRICT>(t, u, k.qy1, k.qy2, delta, tan_delta, fz, rStat, vsx, vsy, omega, fx, fy, My);
 }
 
-// One half-implicit step (rhsFun order, dof18_halfImplicit.cpp:618-673; update :245-318).
-// Tire right-hand-side outputs of one step (before the front forces are rotated), captured for the
-// adjoint's reverse sweep.
-struct TireOut18 {
-    double fx[4], fy[4], My[4], rStat[4], xedot[4], yedot[4];
-};
-
-// STRICT: the reference's own quotients and transcendentals throughout (see dev_tire.h).
 // The step's last part, the position update from the new velocities at the step's yaw angle, then
 // the yaw angle (dof18_halfImplicit.cpp:301-305).  Nothing else in a step reads x, y or psi: a step
 // loop may leave it out (POS = false) and apply it at the start of the next step or before the state
@@ -121,6 +113,15 @@ LFDM_HD void position18(S18& s, double dt) {
     s.psi = s.psi + s.wz * dt;
 }
 
+// Tire right-hand-side outputs of one step (before the front forces are rotated), captured for the
+// adjoint's reverse sweep.
+struct TireOut18 {
+    double fx[4], fy[4], My[4], rStat[4], xedot[4], yedot[4];
+};
+
+// One half-implicit step (rhsFun order, dof18_halfImplicit.cpp:618-673; update :245-318).
+// STRICT: the reference's own quotients and transcendentals throughout (see dev_tire.h).  POS = false
+// leaves out the closing position update (position18).
 template <bool NR, bool FULL, bool CAPTURE = false, bool STRICT = false, bool POS = true>
 LFDM_HD void step18(const PSet& P, const Derived& u, const K18& k, S18& s, double steering,
                                        double throttle, double braking, double dt, X18* x,

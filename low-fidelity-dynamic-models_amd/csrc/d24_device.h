@@ -86,16 +86,6 @@ LFDM_HD Kin24 kin24(const PSet& P, const S24& s, int i) {
     return q;
 }
 
-// One half-implicit step (rhsFun order, dof24_halfImplicit.cpp:594-706; update :188-297).
-// STRICT: the reference's own quotients and transcendentals throughout (see dev_tire.h): the static
-// radius divided by cos(theta) cos(phi), the jacking forces by the track widths, the Euler-angle
-// rates by cos(theta) with tan(theta) from the libm.  Where a reference function uses only the sine
-// or only the cosine of an angle (computeTireRHS: cos theta and cos phi; computeForcesThroughSus:
-// sin phi; the Euler-angle rates of the step: cos theta), the strict step calls that function alone;
-// everywhere else the reference's sin and cos of one angle compile to one sincos, as here.
-// Each strut's kinematics (vehToSusTransform) are computed where a stage uses them: before its tire
-// and again after the tires, from the same state and with the same operations (the same values),
-// so they are not held in registers across the four tire evaluations.
 // The step's last part, the position update from the new velocities and yaw angle
 // (dof24_halfImplicit.cpp:275-280).  Nothing else in a step reads x and y, and psi only here: a step
 // loop may leave it out (POS = false) and apply it at the start of the next step or before the state
@@ -109,6 +99,17 @@ LFDM_HD void position24(S24& s, double dt) {
     s.y = s.y + (s.u * sp + s.v * cp) * dt;
 }
 
+// One half-implicit step (rhsFun order, dof24_halfImplicit.cpp:594-706; update :188-297).
+// STRICT: the reference's own quotients and transcendentals throughout (see dev_tire.h): the static
+// radius divided by cos(theta) cos(phi), the jacking forces by the track widths, the Euler-angle
+// rates by cos(theta) with tan(theta) from the libm.  Where a reference function uses only the sine
+// or only the cosine of an angle (computeTireRHS: cos theta and cos phi; computeForcesThroughSus:
+// sin phi; the Euler-angle rates of the step: cos theta), the strict step calls that function alone;
+// everywhere else the reference's sin and cos of one angle compile to one sincos, as here.
+// Each strut's kinematics (vehToSusTransform) are computed where a stage uses them: before its tire
+// and again after the tires, from the same state and with the same operations (the same values),
+// so they are not held in registers across the four tire evaluations.
+// POS = false leaves out the closing position update (position24).
 template <bool NR, bool FULL, bool STRICT = false, bool POS = true>
 LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, double steering,
                                        double throttle, double braking, double dt, X24* x) {
