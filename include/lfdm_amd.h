@@ -270,9 +270,9 @@ int lfdm_fleet_add(lfdm_fleet* f, lfdm_solver* h);
  * caller never chose one uses a non-blocking stream of its own. */
 int lfdm_fleet_set_stream(lfdm_fleet* f, void* stream);
 /* Launch configuration of a bucket: threads per block (64..256, a multiple of 64), steps per launch
- * and the kernel's waves per SIMD (0: one when the bucket fills at most one wavefront per SIMD, else
- * the model's default, the largest: 1 to 3 for the 11-DOF kernels, 1 or 2 for the 18-DOF ones, 1 for
- * the 24-DOF one). */
+ * and the kernel's waves per SIMD (0: the number of wavefronts per SIMD the bucket fills, up to the
+ * model's default, the largest: 1 to 4 for the 11-DOF kernels, 1 to 3 for the 18-DOF ones, 1 or 2 for
+ * the 24-DOF ones). */
 int lfdm_fleet_set_config(lfdm_fleet* f, int bucket, int block, int steps_per_launch, int waves_per_simd);
 int lfdm_fleet_get_config(lfdm_fleet* f, int bucket, int* block, int* steps_per_launch, int* waves_per_simd,
                           int* layout);

@@ -71,12 +71,20 @@ hipError_t launch_functor_at(const lfdm::Launch& L, int block, int grid, hipStre
 }
 
 // The library's choice of kernel (lfdm::kern::launch_model): the strict step at the model's default
-// occupancy; the fast step at one wave per SIMD for fleets of at most one wavefront per SIMD, else at the
-// model's default.
+// occupancy; the fast step at the occupancy the launch names (L.waves: the wavefronts per SIMD of a
+// fleet that fills fewer than the default), else at the model's default.
 template <class M, bool NR, class F>
 hipError_t launch_functor_tire(const lfdm::Launch& L, int block, int grid, hipStream_t st, const F& f) {
     if (L.strict) return launch_functor_at<M, NR, M::kWaves, true>(L, block, grid, st, f);
-    if (L.waves == 1) return launch_functor_at<M, NR, 1, false>(L, block, grid, st, f);
+    if constexpr (M::kWaves > 3) {
+        if (L.waves == 3) return launch_functor_at<M, NR, 3, false>(L, block, grid, st, f);
+    }
+    if constexpr (M::kWaves > 2) {
+        if (L.waves == 2) return launch_functor_at<M, NR, 2, false>(L, block, grid, st, f);
+    }
+    if constexpr (M::kWaves > 1) {
+        if (L.waves == 1) return launch_functor_at<M, NR, 1, false>(L, block, grid, st, f);
+    }
     return launch_functor_at<M, NR, M::kWaves, false>(L, block, grid, st, f);
 }
 

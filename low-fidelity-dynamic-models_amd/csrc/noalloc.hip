@@ -173,7 +173,11 @@ extern "C" int lfdm_integrate_noalloc(int dof, const double* flat, int len, int 
         L.n_user = (int)n;
         L.mode = kCtrlSteps;
         L.nr = tire_type == LFDM_TIRE_TMEASYNR;
-        L.waves = (S / kWave <= simds) ? 1 : 0;  // the handle path's rule
+        {  // the handle path's rule: the wavefronts per SIMD the fleet fills, below the model's default
+            const long long per = simds > 0 ? ((long long)(S / kWave) + simds - 1) / simds : 0;
+            const int wmax = dof == 11 ? kMaxWaves11 : dof == 18 ? kMaxWaves18 : kMaxWaves24;
+            L.waves = (per > 0 && per < wmax) ? (int)per : 0;
+        }
         L.strict = arith == LFDM_ARITH_STRICT;
         const hipError_t e = dof == 11 ? launch_d11(L, kWave, st) : dof == 18 ? launch_d18(L, kWave, st)
                                                                             : launch_d24(L, kWave, st);

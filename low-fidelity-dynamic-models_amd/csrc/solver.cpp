@@ -437,6 +437,8 @@ std::vector<int> output_steps(int nsteps, double step, double dtout) {
 }
 
 // SIMDs of the current device (4 per CU), queried once per handle.
+int max_waves(int dof);
+
 int simd_count(lfdm_solver* h) {
     if (h->simds > 0) return h->simds;
     int dev = 0, cus = 0;
@@ -480,9 +482,12 @@ Launch make_launch(lfdm_solver* h, int mode, double t0, int nsteps, double cs = 
         L.wave_hmax = h->d_wave_hmax;
         L.step_base = (int)std::llround(t0 / h->step);
     }
-    // a fleet of at most one wavefront per SIMD runs the kernels built for one wave per SIMD
+    // a fleet of at most w wavefronts per SIMD, w below the model's default, runs the kernels built for
+    // w waves per SIMD: their larger register budget spills less, and no SIMD can take more of the
+    // fleet's wavefronts than w (at two per SIMD, the 18-DOF kernel built for three ran 1.3x slower)
     const int simds = simd_count(h);
-    L.waves = (simds > 0 && h->S / kWave <= simds) ? 1 : 0;
+    const long long per = simds > 0 ? ((long long)h->S / kWave + simds - 1) / simds : 0;
+    L.waves = (per > 0 && per < max_waves(h->dof)) ? (int)per : 0;
     L.strict = h->arith == LFDM_ARITH_STRICT ? 1 : 0;
     return L;
 }
