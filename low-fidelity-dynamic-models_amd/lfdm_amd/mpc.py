@@ -52,6 +52,18 @@ def tracking_loss(xy, ref, dt=1e-3):
     return dt * (d * d).sum(dim=(0, 1))
 
 
+def tracking_loss_and_grad(xy, ref, dt=1e-3):
+    """tracking_loss summed over the vehicles (a scalar tensor) and its gradient with respect to xy,
+    2 dt (xy - ref), in three passes over the trajectory (difference, dot product, in-place scaling);
+    the gradient has the same bits as autograd's of tracking_loss(xy, ref).sum() (doubling is exact)."""
+    with torch.no_grad():
+        d = xy.detach() - ref
+        v = d.view(-1)
+        loss = dt * torch.dot(v, v)
+        d.mul_(2.0 * dt)
+    return loss, d
+
+
 def initial_controls(flat, n, nsteps, kappa, device="cuda"):
     """Feed-forward guess: kinematic-bicycle steering for the arc's curvature, throttle 0.3."""
     wheelbase = flat[W.FLAT18["a"]] + flat[W.FLAT18["b"]]
@@ -61,10 +73,12 @@ def initial_controls(flat, n, nsteps, kappa, device="cuda"):
     return c
 
 
-def run_mpc(solver, flat, init, params, ref, ctrl0, iterations=20, lr=0.05, checkpoint_every=100):
+def run_mpc(solver, flat, init, params, ref, ctrl0, iterations=20, lr=0.05, checkpoint_every=100, dt=1e-3):
     """Adam on the controls; returns (controls, per-iteration mean loss).  params and init move to the
     controls' device once; the iterations queue work without waiting for the GPU (the losses are read
-    back after the last one)."""
+    back after the last one).  The loss and its gradient with respect to the trajectory come from
+    tracking_loss_and_grad (three passes over the (steps, 2, vehicles) trajectory instead of
+    autograd's graph of the loss)."""
     ctrl = ctrl0.clone().requires_grad_(True)
     params = params.to(ctrl.device, torch.float64)
     init = init.to(ctrl.device, torch.float64)
@@ -79,8 +93,8 @@ def run_mpc(solver, flat, init, params, ref, ctrl0, iterations=20, lr=0.05, chec
     for _ in range(iterations):
         opt.zero_grad()
         xy = rollout(solver, ctrl, params, init, checkpoint_every)
-        loss = tracking_loss(xy, ref).sum()
-        loss.backward()
+        loss, g = tracking_loss_and_grad(xy, ref, dt)
+        xy.backward(g)
         opt.step()
         with torch.no_grad():
             torch.clamp(ctrl, lo, hi, out=ctrl)

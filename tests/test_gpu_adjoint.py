@@ -232,7 +232,8 @@ def test_adjoint_matches_dual_oracle_on_tracking_workload():
 def test_torch_rollout_backward_and_mpc_step():
     """rollout() is a torch.autograd.Function: loss.backward() fills controls.grad, params.grad and
     init.grad; gradients are linear in grad_output; a few MPC iterations reduce the tracking loss."""
-    from lfdm_amd.mpc import arc_reference, initial_controls, moving_start, run_mpc, tracking_loss
+    from lfdm_amd.mpc import arc_reference, initial_controls, moving_start, run_mpc, tracking_loss, \
+        tracking_loss_and_grad
     from lfdm_amd.torch_ops import default_params, make_solver, rollout
 
     flat = hmmwv()
@@ -256,6 +257,13 @@ def test_torch_rollout_backward_and_mpc_step():
     xy = rollout(s, ctrl, p, i0, 100)
     (2.0 * tracking_loss(xy, ref).sum()).backward()
     assert torch.equal(ctrl.grad, 2.0 * g1)
+    # the MPC loop's loss gradient formed directly is autograd's, bit for bit
+    ctrl.grad = None
+    xy = rollout(s, ctrl, p, i0, 100)
+    l2, g = tracking_loss_and_grad(xy, ref)
+    xy.backward(g)
+    assert torch.equal(ctrl.grad, g1)
+    assert abs(l2.item() - loss.item()) <= 1e-12 * abs(loss.item())
     # checkpoint interval does not change the result
     ctrl.grad = None
     xy = rollout(s, ctrl, p, i0, 37)
