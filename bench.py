@@ -506,11 +506,11 @@ def mixed_fleet(n_total=2_000_000, split=(0.4, 0.4, 0.2), tmin=2.0, tmax=20.0, j
             "speedup_vs_grouped_eager": eager_wall / wall}
 
 
-def counters_figure(key):
-    """A figure of the 18-DOF integration kernel's counter runs summarized in profiles/ (None if the
+def counters_figure(key, dof=18):
+    """A figure of a model's integration-kernel counter runs summarized in profiles/ (None if the
     summary is absent): "traffic_bytes_per_vehicle_step" (calibrated FETCH_SIZE + WRITE_SIZE) or
     "valu_busy_per_simd" (SQ_ACTIVE_INST_VALU / SQ_WAVE_CYCLES x waves per SIMD)."""
-    path = os.path.join(ROOT, "profiles", "integrate_18dof_knot_fleet_counters.md")
+    path = os.path.join(ROOT, "profiles", f"integrate_{dof}dof_knot_fleet_counters.md")
     try:
         for line in open(path):
             if line.startswith(key + ":"):
@@ -646,7 +646,11 @@ def main():
                                  "steps": 2, "kernel_ms": k_, "flop_per_vehicle_step": f_,
                                  "flop_per_vehicle_step_distinct_libm": fe_, "basic_flop_per_vehicle_step": fb_,
                                  "fp64_frac": tf_(f_), "fp64_frac_distinct_libm": tf_(fe_),
-                                 "fp64_frac_basic_ops": tf_(fb_)}
+                                 "fp64_frac_basic_ops": tf_(fb_),
+                                 "valu_busy_per_simd": counters_figure("valu_busy_per_simd", d_),
+                                 "traffic_bytes_per_vehicle_step": counters_figure("traffic_bytes_per_vehicle_step", d_),
+                                 "counters_source": (f"profiles/integrate_{d_}dof_knot_fleet_counters.md"
+                                                     if counters_figure("valu_busy_per_simd", d_) is not None else None)}
     adj = None
     if dof == 18 and not args.no_adjoint:
         adj = adjoint_cost(flat, mpc_iterations=args.mpc_iterations)
