@@ -15,12 +15,12 @@ namespace dev {
 // Carried state.  Per tire LF RF LR RR: deflections (TMeasy), compression xt, wheel speed and the
 // static radius of the last step (read by vehToTireTransform, dof24.cpp:340-364); per strut:
 // deflection xs, unsprung heave velocity wu, last deflection rate dxs and damper transfer force fzd
-// (read by computeSusRHS, :1199-1228) plus the constants xsi, lsi and the tire's vsz set by
-// Initialize; chassis velocities and angles with the accelerations of the last step (read by
-// vehToSusTransform, :199-229).
+// (read by computeSusRHS, :1199-1228); chassis velocities and angles with the accelerations of the
+// last step (read by vehToSusTransform, :199-229).  The record's per-strut values that Initialize
+// sets and no step changes (xsi, lsi, the tire's vsz) are in K24.
 struct S24 {
     double xe[4], ye[4], xedot[4], yedot[4], xt[4], omega[4], rStat[4];
-    double xs[4], wu[4], dxs[4], fzd[4], xsi[4], lsi[4], vsz[4];
+    double xs[4], wu[4], dxs[4], fzd[4];
     double x, y, u, v, w, psi, phi, theta, wx, wy, wz;
     double udot, vdot, wdot, wxdot, wydot, wzdot, crank;
     int gear;
@@ -35,8 +35,12 @@ struct X24 {
     double dtheta, dphi, dpsi, dcrank;
 };
 
-// Per-vehicle constants.
+// Per-vehicle constants: from the parameters and overrides (k24_init), and the record's per-strut
+// values no step changes (initial strut deflection xsi and length lsi, tire compression velocity vsz;
+// read from the record with the state), which the integration kernel keeps with the rest in LDS
+// rather than in registers.
 struct K24 {
+    double xsi[4], lsi[4], vsz[4];
     double m, tqs, maxBrake, qy1, qy2;
     double inv_m, inv_jx, inv_jy, inv_jz, mu_g[2], inv_mu[2], half_cf, half_cr;
 };
@@ -64,7 +68,7 @@ struct Kin24 {
     double us, vs, ls, uu, vu;
 };
 
-LFDM_HD Kin24 kin24(const PSet& P, const S24& s, int i) {
+LFDM_HD Kin24 kin24(const PSet& P, const S24& s, const K24& k, int i) {
     const double a = P.a, b = P.b, cf = P.cf, cr = P.cr;
     Kin24 q;
     if (i == 0) {
@@ -80,7 +84,7 @@ LFDM_HD Kin24 kin24(const PSet& P, const S24& s, int i) {
         q.us = (cr * s.wz / 2.) + s.u;
         q.vs = -b * s.wz + s.v;
     }
-    q.ls = s.lsi[i] - (s.xs[i] - s.xsi[i]);
+    q.ls = k.lsi[i] - (s.xs[i] - k.xsi[i]);
     q.uu = q.us - q.ls * s.wy;
     q.vu = q.vs + q.ls * s.wx;
     return q;
@@ -134,7 +138,7 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     const double rcc = frcp(cc);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const Kin24 q = kin24(P, s, i);
+        const Kin24 q = kin24(P, s, k, i);
         const double vgx = cth * (q.uu - s.wy * s.rStat[i]) + sth * (s.wu[i] * cph + sph * (s.wx * s.rStat[i] + q.vu));
         const double vgy = cph * (q.vu + s.wx * s.rStat[i]) - s.wu[i] * sph;
         const double vsy = vgy;
@@ -165,7 +169,7 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     double ls[4], uu[4], vu[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const Kin24 q = kin24(P, s, i);
+        const Kin24 q = kin24(P, s, k, i);
         ls[i] = q.ls;
         uu[i] = q.uu;
         vu[i] = q.vu;
@@ -178,7 +182,7 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
     // its old value, with the same operations as an update at the end of the step.
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const double dxt = s.vsz[i] - (cth * (s.wu[i] * cph + vu[i] * sph) - uu[i] * sth);
+        const double dxt = k.vsz[i] - (cth * (s.wu[i] * cph + vu[i] * sph) - uu[i] * sth);
         if (FULL) x->dxt[i] = dxt;
         if (!NR) {
             s.xe[i] = s.xe[i] + s.xedot[i] * dt;

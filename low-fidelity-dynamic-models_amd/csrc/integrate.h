@@ -103,7 +103,7 @@ struct M18 {
         static LFDM_HD int f(int i, int o) { return i * kTire + om + o; }  // o: omega = 0 ...
     };
     template <bool NR>
-    static LFDM_HD void load(const double* st, size_t S, int slot, S18& s) {
+    static LFDM_HD void load(const double* st, size_t S, int slot, S18& s, K18&) {
         using Rr = R<NR>;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -178,7 +178,7 @@ struct M11 {
         static LFDM_HD int f(int i, int o) { return i * kTire + om + o; }
     };
     template <bool NR>
-    static LFDM_HD void load(const double* st, size_t S, int slot, S11& s) {
+    static LFDM_HD void load(const double* st, size_t S, int slot, S11& s, K11&) {
         using Rr = R<NR>;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
@@ -256,7 +256,7 @@ struct M24 {
     // tire fields after omega: dOmega 1, xt 2, dxt 3, rStat 4, fx 5, fy 6, fz 7, fxg 8, fyg 9, fzg 10,
     // vgx 11, vgy 12, vgz 13, vsx 14, vsy 15, vsz 16, My 17, engTor 18
     template <bool NR>
-    static LFDM_HD void load(const double* st, size_t S, int slot, S24& s) {
+    static LFDM_HD void load(const double* st, size_t S, int slot, S24& s, K24& k) {
         using Rr = R<NR>;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -271,11 +271,11 @@ struct M24 {
             s.omega[i] = st[Rr::f(i, 0) * S + slot];
             s.xt[i] = st[Rr::f(i, 2) * S + slot];
             s.rStat[i] = st[Rr::f(i, 4) * S + slot];
-            s.vsz[i] = st[Rr::f(i, 16) * S + slot];
+            k.vsz[i] = st[Rr::f(i, 16) * S + slot];
             s.xs[i] = st[Rr::q(i, 0) * S + slot];
-            s.xsi[i] = st[Rr::q(i, 1) * S + slot];
+            k.xsi[i] = st[Rr::q(i, 1) * S + slot];
             s.dxs[i] = st[Rr::q(i, 2) * S + slot];
-            s.lsi[i] = st[Rr::q(i, 4) * S + slot];
+            k.lsi[i] = st[Rr::q(i, 4) * S + slot];
             s.wu[i] = st[Rr::q(i, 13) * S + slot];
             s.fzd[i] = st[Rr::q(i, 20) * S + slot];
         }
@@ -375,14 +375,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
     const PSet& P = pset_at(L.sets, set);
     const size_t S = (size_t)L.S;
 
-    typename M::S s;
-    M::template load<NR>(L.state, S, slot, s);
-    const Derived& u = P.d;
     // the per-vehicle constants: in this lane's LDS slot (M::kKInLds; the launch sizes the dynamic LDS
     // with k_lds_bytes), read where the step uses them, or in registers
     extern __shared__ double lfdm_k_lds[];
     typename M::K k_reg;
     typename M::K& k = M::kKInLds ? reinterpret_cast<typename M::K*>(lfdm_k_lds)[threadIdx.x] : k_reg;
+    typename M::S s;
+    M::template load<NR>(L.state, S, slot, s, k);
+    const Derived& u = P.d;
     M::kinit(P, L.ovr + slot, S, k);
     const int user = L.slot_user[slot];
     const int uv = user >= 0 ? user : 0;  // padding lanes read vehicle 0's controls

@@ -33,8 +33,8 @@ void run(const PSet& P, const double* ovr, const double* init, int n, int rec, i
         for (int i = 0; i < n; ++i) ov[(size_t)f * S + i] = ovr ? ovr[(size_t)f * n + i] : dflt[f];
     for (int i = 0; i < n; ++i) {
         typename M::S s;
-        M::template load<NR>(st.data(), S, i, s);
         typename M::K k;
+        M::template load<NR>(st.data(), S, i, s, k);
         M::kinit(P, ov.data() + i, S, k);
         typename M::X x;
         for (int j = 0; j < nsteps; ++j) {
