@@ -17,9 +17,9 @@ namespace dev {
 // deflection xs, unsprung heave velocity wu, last deflection rate dxs and damper transfer force fzd
 // (read by computeSusRHS, :1199-1228); chassis velocities and angles with the accelerations of the
 // last step (read by vehToSusTransform, :199-229).  The record's per-strut values that Initialize
-// sets and no step changes (xsi, lsi, the tire's vsz) are in K24.
+// sets and no step changes (xsi, lsi, the tire's vsz) and the tires' deflection rates are in K24.
 struct S24 {
-    double xe[4], ye[4], xedot[4], yedot[4], xt[4], omega[4], rStat[4];
+    double xe[4], ye[4], xt[4], omega[4], rStat[4];
     double xs[4], wu[4], dxs[4], fzd[4];
     double x, y, u, v, w, psi, phi, theta, wx, wy, wz;
     double udot, vdot, wdot, wxdot, wydot, wzdot, crank;
@@ -35,12 +35,13 @@ struct X24 {
     double dtheta, dphi, dpsi, dcrank;
 };
 
-// Per-vehicle constants: from the parameters and overrides (k24_init), and the record's per-strut
-// values no step changes (initial strut deflection xsi and length lsi, tire compression velocity vsz;
-// read from the record with the state), which the integration kernel keeps with the rest in LDS
-// rather than in registers.
+// Per-vehicle values the integration kernel keeps in LDS rather than in registers: the constants from
+// the parameters and overrides (k24_init), the record's per-strut values no step changes (initial strut
+// deflection xsi and length lsi, tire compression velocity vsz), and the TMeasy deflection rates xedot,
+// yedot (state, but written once per step by the tire and read once by the update); the last two
+// groups are read from the record with the state.
 struct K24 {
-    double xsi[4], lsi[4], vsz[4];
+    double xsi[4], lsi[4], vsz[4], xedot[4], yedot[4];
     double m, tqs, maxBrake, qy1, qy2;
     double inv_m, inv_jx, inv_jy, inv_jz, mu_g[2], inv_mu[2], half_cf, half_cr;
 };
@@ -115,7 +116,7 @@ LFDM_HD void position24(S24& s, double dt) {
 // so they are not held in registers across the four tire evaluations.
 // POS = false leaves out the closing position update (position24).
 template <bool NR, bool FULL, bool STRICT = false, bool POS = true>
-LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, double steering,
+LFDM_HD void step24(const PSet& P, const Derived& u, K24& k, S24& s, double steering,
                                        double throttle, double braking, double dt, X24* x) {
     const TireP& t = P.tire;
     const double a = P.a, b = P.b, cf = P.cf, cr = P.cr;
@@ -157,7 +158,7 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
         }
         if (!NR)
             tmeasy_core<STRICT>(t, u, k.qy1, k.qy2, dl, d, fz[i], s.rStat[i], vsx, vsy, s.xe[i], s.ye[i],
-                                s.omega[i], s.xedot[i], s.yedot[i], fx[i], fy[i], My[i]);
+                                s.omega[i], k.xedot[i], k.yedot[i], fx[i], fy[i], My[i]);
         else
             tmeasy_nr_core<STRICT>(t, u, k.qy1, k.qy2, dl, d, fz[i], s.rStat[i], vsx, vsy, s.omega[i], fx[i],
                                    fy[i], My[i]);
@@ -185,8 +186,8 @@ LFDM_HD void step24(const PSet& P, const Derived& u, const K24& k, S24& s, doubl
         const double dxt = k.vsz[i] - (cth * (s.wu[i] * cph + vu[i] * sph) - uu[i] * sth);
         if (FULL) x->dxt[i] = dxt;
         if (!NR) {
-            s.xe[i] = s.xe[i] + s.xedot[i] * dt;
-            s.ye[i] = s.ye[i] + s.yedot[i] * dt;
+            s.xe[i] = s.xe[i] + k.xedot[i] * dt;
+            s.ye[i] = s.ye[i] + k.yedot[i] * dt;
         }
         s.xt[i] = s.xt[i] + dxt * dt;
     }

@@ -138,7 +138,7 @@ struct M18 {
         o[9 * nu] = s.omega[0]; o[10 * nu] = s.omega[1]; o[11 * nu] = s.omega[2]; o[12 * nu] = s.omega[3];
     }
     template <bool NR>
-    static LFDM_HD void store(double* w, size_t S, int slot, const PSet& P, const S18& s,
+    static LFDM_HD void store(double* w, size_t S, int slot, const PSet& P, const S18& s, const K18&,
                                                  const X18& x) {
         using Rr = R<NR>;
 #pragma unroll
@@ -212,7 +212,7 @@ struct M11 {
         o[5 * nu] = s.psi; o[6 * nu] = s.wz; o[7 * nu] = s.omega[0]; o[8 * nu] = s.omega[1];
     }
     template <bool NR>
-    static LFDM_HD void store(double* w, size_t S, int slot, const PSet& P, const S11& s,
+    static LFDM_HD void store(double* w, size_t S, int slot, const PSet& P, const S11& s, const K11&,
                                                  const X11& x) {
         using Rr = R<NR>;
 #pragma unroll
@@ -263,10 +263,10 @@ struct M24 {
             if (!NR) {
                 s.xe[i] = st[(i * Rr::kTire + 0) * S + slot];
                 s.ye[i] = st[(i * Rr::kTire + 1) * S + slot];
-                s.xedot[i] = st[(i * Rr::kTire + 2) * S + slot];
-                s.yedot[i] = st[(i * Rr::kTire + 3) * S + slot];
+                k.xedot[i] = st[(i * Rr::kTire + 2) * S + slot];
+                k.yedot[i] = st[(i * Rr::kTire + 3) * S + slot];
             } else {
-                s.xe[i] = s.ye[i] = s.xedot[i] = s.yedot[i] = 0.0;
+                s.xe[i] = s.ye[i] = k.xedot[i] = k.yedot[i] = 0.0;
             }
             s.omega[i] = st[Rr::f(i, 0) * S + slot];
             s.xt[i] = st[Rr::f(i, 2) * S + slot];
@@ -290,7 +290,7 @@ struct M24 {
         k24_init(P, o[0], o[S], o[2 * S], o[3 * S], o[4 * S], o[5 * S], o[6 * S], k);
     }
     template <bool NR, bool FULL, bool STRICT, bool POS = true>
-    static LFDM_HD void step(const PSet& P, const Derived& u, const K24& k, S24& s, const Ctl& c,
+    static LFDM_HD void step(const PSet& P, const Derived& u, K24& k, S24& s, const Ctl& c,
                                                 double dt, X24* x) {
         step24<NR, FULL, STRICT, POS>(P, u, k, s, c.s, c.th, c.b, dt, x);
     }
@@ -303,7 +303,7 @@ struct M24 {
         o[14 * nu] = s.omega[3];
     }
     template <bool NR>
-    static LFDM_HD void store(double* w, size_t S, int slot, const PSet& P, const S24& s,
+    static LFDM_HD void store(double* w, size_t S, int slot, const PSet& P, const S24& s, const K24& k,
                                                  const X24& x) {
         using Rr = R<NR>;
 #pragma unroll
@@ -311,8 +311,8 @@ struct M24 {
             if (!NR) {
                 w[(i * Rr::kTire + 0) * S + slot] = s.xe[i];
                 w[(i * Rr::kTire + 1) * S + slot] = s.ye[i];
-                w[(i * Rr::kTire + 2) * S + slot] = s.xedot[i];
-                w[(i * Rr::kTire + 3) * S + slot] = s.yedot[i];
+                w[(i * Rr::kTire + 2) * S + slot] = k.xedot[i];
+                w[(i * Rr::kTire + 3) * S + slot] = k.yedot[i];
             }
             // omega .. vgy (vgz, vsz are never written by the model), vsx vsy My engTor
             const double f[13] = {s.omega[i], x.dOmega[i], s.xt[i],   x.dxt[i],  s.rStat[i], x.fx[i], x.fy[i],
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) voi
         M::template step<NR, true, STRICT, false>(P, u, k, s, c, dt, &x);
         M::template position<STRICT>(s, dt);
         t = t + dt;
-        M::template store<NR>(L.state, S, slot, P, s, x);
+        M::template store<NR>(L.state, S, slot, P, s, k, x);
     }
     // rows at or after the vehicle's last step: its final state, at the row's time
     if (user >= 0) {

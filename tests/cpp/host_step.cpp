@@ -45,7 +45,7 @@ void run(const PSet& P, const double* ovr, const double* init, int n, int rec, i
             else
                 M::template step<NR, true, STRICT>(P, P.d, k, s, cc, dt, &x);
         }
-        if (nsteps > 0) M::template store<NR>(st.data(), S, i, P, s, x);
+        if (nsteps > 0) M::template store<NR>(st.data(), S, i, P, s, k, x);
     }
     for (int i = 0; i < n; ++i)
         for (int f = 0; f < rec; ++f) fin[(size_t)i * rec + f] = st[(size_t)f * S + i];
