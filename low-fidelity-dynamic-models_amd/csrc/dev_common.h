@@ -216,8 +216,7 @@ LFDM_HD unsigned int two_over_pi_word(int k) {
 // (224 bits) are multiplied by M exactly in 32-bit limbs, and the product's 128 bits from two above
 // the binary point down give n mod 4 and f to 126 bits (the words left out weigh below 2^-170 in
 // x 2/pi, far below the 2^-62 of the closest double to a multiple of pi/2).  f goes to
-// double-double and is multiplied by pi/2 in two parts.  Integer arithmetic only up to there, in
-// about thirty registers.
+// double-double and is multiplied by pi/2 in two parts.  Integer arithmetic only up to there.
 LFDM_HD int reduce_pio2_large(double x, double& rh, double& rl) {
     const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
     const int e = (int)((b >> 52) & 0x7ff) - 1075;
