@@ -475,6 +475,22 @@ def mixed_fleet(n_total=2_000_000, split=(0.4, 0.4, 0.2), tmin=2.0, tmax=20.0, j
         walls.append(time.perf_counter() - t0)
         ms.append(f.last_ms())
     wall = min(walls)
+    f.close()
+    # comparison: the same buckets one after another, each a graph of its own with its autotuned
+    # configuration and layout (what running the three models' fleets separately takes)
+    seq = []
+    for b, s in enumerate(buckets):
+        g = Fleet([s])
+        g.set_config(0, cfg[b][0], cfg[b][1], cfg[b][2])
+        reset()
+        g.run(T)  # captures the graph
+        g.synchronize()
+        reset()
+        t0 = time.perf_counter()
+        g.run(T)
+        g.synchronize()
+        seq.append(time.perf_counter() - t0)
+        g.close()
     # comparison: vehicle order, one eager launch per model on each handle's own stream (the device
     # image in vehicle order is built, by a one-step launch, before the records are reset and timed)
     for s in buckets:
@@ -489,7 +505,6 @@ def mixed_fleet(n_total=2_000_000, split=(0.4, 0.4, 0.2), tmin=2.0, tmax=20.0, j
     for s in buckets:
         s.synchronize()
     eager_wall = time.perf_counter() - t0
-    f.close()
     for s in buckets:
         s.close()
     return {"workload": f"mixed: {n_total:,} vehicles 11/18/24-DOF {split[0]:.0%}/{split[1]:.0%}/{split[2]:.0%}, "
@@ -502,6 +517,8 @@ def mixed_fleet(n_total=2_000_000, split=(0.4, 0.4, 0.2), tmin=2.0, tmax=20.0, j
                                      for d_, h, c in zip((11, 18, 24), horizons, cfg)]},
             "lane_efficiency": {"by_horizon": [lane_efficiency(h, True) for h in horizons],
                                 "grouped": [lane_efficiency(h, False) for h in horizons]},
+            "buckets_one_after_another": {"wall_s": sum(seq), "per_bucket_s": seq,
+                                          "graph_speedup": sum(seq) / wall},
             "grouped_eager": {"value": work / eager_wall, "wall_s": eager_wall},
             "speedup_vs_grouped_eager": eager_wall / wall}
 

@@ -186,6 +186,36 @@ def test_fleet_graph_matches_eager_runs():
 MAX_WAVES = {11: 4, 18: 3, 24: 2}
 
 
+def test_large_fleet_graph_chains_buckets_and_matches_eager_runs():
+    """Buckets that each fill the chip (on MI355X: at least 1,024 SIMDs x the kernel's waves per SIMD
+    wavefronts) run one after another in the fleet's graph instead of side by side; with horizons,
+    two replays, each bucket bit-identical to its own eager run."""
+    T = 120
+    sizes = ((11, 270_000, 5), (18, 200_000, 6), (24, 140_000, 7))
+    say(f"[batching] large fleet graph ({', '.join(f'{n:,} {d}-DOF' for d, n, _ in sizes)}) vs eager, {T} steps")
+    a, ref = [], []
+    for dof, n, seed in sizes:
+        hz = np.random.default_rng(seed).integers(T // 2, T + 1, n)
+        a.append(build(dof, n, seed=seed, horizons=hz))
+        r = build(dof, n, seed=seed, horizons=hz)
+        r.advance(T)
+        ref.append(r.get_states())
+        r.close()
+    init = [s.get_states() for s in a]
+    f = Fleet(a)
+    for rep in range(2):
+        for s, i0 in zip(a, init):
+            s.set_states(i0)
+        f.run(T)
+        f.synchronize()
+        say(f"[batching] large fleet replay {rep}: {f.last_ms():.2f} ms")
+        for (dof, _, _), s, r_ in zip(sizes, a, ref):
+            np.testing.assert_array_equal(s.get_states(), r_, err_msg=f"replay {rep}, {dof}-DOF")
+    f.close()
+    for s in a:
+        s.close()
+
+
 @pytest.mark.parametrize("dof", [11, 18, 24])
 def test_every_occupancy_gives_the_same_bits(dof):
     """The kernels built for one up to the model's most waves per SIMD run the same arithmetic: a fleet
