@@ -256,8 +256,10 @@ int lfdm_get_layout(lfdm_solver* h, int* layout);
 /* A mixed fleet: solver handles of any models (buckets, normally one per model, so a wavefront never
  * mixes models) integrated together.  lfdm_fleet_run(f, nsteps) does for every bucket what Solve
  * does without output: nsteps steps from t = 0 from the current records with each bucket's control
- * source (driver tables, knots or per-step arrays from row 0) and horizons, as one HIP graph with a
- * branch per bucket, each a chain of launches of steps_per_launch steps (0 = one launch).  The graph
+ * source (driver tables, knots or per-step arrays from row 0) and horizons, as one HIP graph of one
+ * chain of launches of steps_per_launch steps (0 = one launch) per bucket: the chains run one after
+ * another when every bucket fills the GPU by itself (at least as many wavefronts as its kernels'
+ * occupancy lets the GPU hold at once), else side by side on branches of the graph.  The graph
  * is captured on the first run and replayed while the launch configuration is unchanged.  Runs are
  * asynchronous on the fleet's stream; the buckets' streams are ordered after them.  The handles
  * stay owned by the caller and must outlive the fleet. */
